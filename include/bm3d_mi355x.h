@@ -1,0 +1,163 @@
+/*
+ * bm3d_mi355x -- C ABI of the MI355X (gfx950) BM3D / V-BM3D denoiser.
+ *
+ * Drop-in replacement for the device seam of the bm3dcuda VapourSynth
+ * plugin. In the reference, the plugin layer (source/source.cpp) talks to
+ * the GPU through one C++ function,
+ *
+ *   cudaGraphExec_t get_graphexec(float *d_res, float *d_src, float *h_res,
+ *       int width, int height, int stride, float sigma, int block_step,
+ *       int bm_range, int radius, int ps_num, int ps_range, bool chroma,
+ *       float sigma_u, float sigma_v, bool final_, float extractor)
+ *                                         (source/kernel.cu:40-53, 759-864)
+ *
+ * which bakes one plane group's parameters into a graph, plus two host loops
+ * around it: the r = 0 division `Aggregation` (source/source.cpp:170-188)
+ * and `VAggregateGetFrame` (source/source.cpp:746-862). This header replaces
+ * all three with device-resident entry points:
+ *
+ *   bm3d_plan_create / bm3d_plan_destroy   <- get_graphexec (parameters baked
+ *                                             into a plan, per plane group)
+ *   bm3d_execute                           <- graph launch (kernel node) for
+ *                                             any number of output frames
+ *   bm3d_execute_window                    <- graph launch on the reference's
+ *                                             packed d_src layout
+ *   bm3d_aggregate                         <- Aggregation (r = 0)
+ *   bm3d_vaggregate                        <- VAggregateGetFrame (r > 0)
+ *
+ * and the plugin-level filters of source/source.cpp:457-989 (BM3D,
+ * VAggregate, BM3Dv2 argument handling) are provided by bm3d_filter_*.
+ *
+ * Conventions: plain pointers and sizes; all float buffers are device
+ * memory (hipMalloc) unless stated; `stream` is a hipStream_t passed as
+ * void* (NULL = default stream); calls are asynchronous on that stream.
+ * Every entry point returns BM3D_OK or a BM3D_E* code and records a message
+ * readable with bm3d_last_error() on the calling thread. No C++ exception
+ * crosses this boundary.
+ */
+#ifndef BM3D_MI355X_H
+#define BM3D_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM3D_OK 0
+#define BM3D_EINVAL 1       /* invalid argument (message says which)       */
+#define BM3D_EHIP 2         /* HIP runtime error                            */
+#define BM3D_EUNSUPPORTED 3 /* valid for the reference, outside this build  */
+
+#define BM3D_PRECISION_FP32 0  /* all FP32 on VALU                          */
+#define BM3D_PRECISION_MIXED 1 /* FP16 MFMA block-matching pre-selection,
+                                  exact FP32 rescoring                      */
+
+typedef struct bm3d_plan bm3d_plan;
+
+/* Parameters of one plane group: one plane, or the three planes of a
+ * YUV444 clip with chroma = 1 (block matching on plane 0). Field meanings
+ * and defaults follow the BM3D filter arguments (source/source.cpp:
+ * 503-629); sigma is in user units and is scaled here exactly as
+ * source/source.cpp:520-523. */
+typedef struct bm3d_desc {
+    int width, height;   /* plane size, >= 8 x 8 */
+    int stride;          /* row pitch in floats of every plane buffer   */
+    float sigma[3];      /* [0]: plane 0; [1], [2]: chroma planes      */
+    int block_step;      /* 1..8 */
+    int bm_range;        /* >= 1 */
+    int radius;          /* 0 = BM3D, > 0 = V-BM3D */
+    int ps_num;          /* 1..8 */
+    int ps_range;        /* >= 1 */
+    int chroma;          /* 0 or 1 */
+    int final_;          /* 0: basic (hard threshold); 1: final (Wiener, needs ref) */
+    int extractor_exp;   /* 0 = off, else pre-rounding with 2^extractor_exp */
+    int precision;       /* BM3D_PRECISION_* */
+} bm3d_desc;
+
+/* A clip in device memory: frame f, plane p starts at
+ * data + f * frame_stride + p * plane_stride; rows are desc.stride apart. */
+typedef struct bm3d_clip {
+    const float *data;
+    long long frame_stride;
+    long long plane_stride;
+    int num_frames;
+} bm3d_clip;
+
+/* Creates a plan on the current HIP device. Validates the parameters with
+ * the reference's messages (source/source.cpp:476-611). */
+int bm3d_plan_create(const bm3d_desc *desc, bm3d_plan **plan);
+void bm3d_plan_destroy(bm3d_plan *plan);
+
+/* Floats in one output stack: planes * (2r+1) * 2 * height * stride
+ * (the reference's d_res shape, source/kernel.cu:41-42). */
+long long bm3d_stack_size(const bm3d_plan *plan);
+
+/* Evaluates BM3D for output frames first_frame .. first_frame+count-1 of
+ * `src` (ref: the basic estimate, required iff desc.final_). Frame n's
+ * temporal slot z is frame clamp(n - r + z, 0, num_frames - 1)
+ * (source/source.cpp:240-264). Output frame i's stack starts at
+ * d_res + i * res_frame_stride; its plane p at + p * res_plane_stride; slot
+ * z of a plane holds `height` rows of the weighted sum, then `height` rows of
+ * the weight (source/source.cpp:362-366), rows desc.stride floats apart.
+ * A stride of 0 means packed (bm3d_stack_size() per frame,
+ * (2r+1) * 2 * height * stride per plane). ref must share src's strides.
+ * The stacks are fully overwritten. */
+int bm3d_execute(bm3d_plan *plan, float *d_res, long long res_frame_stride, long long res_plane_stride,
+                 const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count, void *stream);
+
+/* One window in the reference's packed input layout
+ * d_src = [(final ? 2 : 1)][planes][2r+1][height][stride], ref first
+ * (source/kernel.cu:43-44). Writes one packed stack to d_res. */
+int bm3d_execute_window(bm3d_plan *plan, float *d_res, const float *d_src, void *stream);
+
+/* r = 0 aggregation, dst = num / den per pixel (source/source.cpp:170-188)
+ * for `count` frames of `nplanes` planes. res: stacks with one slot per
+ * plane (strides as in bm3d_execute; 0 = packed). */
+int bm3d_aggregate(float *dst, long long dst_frame_stride, long long dst_plane_stride, int dst_stride,
+                   const float *res, long long res_frame_stride, long long res_plane_stride,
+                   int width, int height, int stride, int nplanes, int count, void *stream);
+
+/* VAggregate (source/source.cpp:746-862) for output frames
+ * first_frame .. first_frame+count-1 of a num_frames-frame clip.
+ * stacks: the stacks of frames stack_first, stack_first+1, ... (strides as
+ * in bm3d_execute; 0 = packed), covering clamp(n - r, 0, N-1) ..
+ * clamp(n + r, 0, N-1) for every output frame n. */
+int bm3d_vaggregate(float *dst, long long dst_frame_stride, long long dst_plane_stride, int dst_stride,
+                    const float *stacks, long long stack_frame_stride, long long stack_plane_stride,
+                    int stack_first, int width, int height, int stride, int nplanes, int radius,
+                    int num_frames, int first_frame, int count, void *stream);
+
+/* Block matching only (no counterpart entry point in the reference; it
+ * exposes the group lists the kernel of source/kernel.cu:432-647 builds in
+ * registers). Writes, for each output frame, nblocks x 8 packed positions
+ * x | y << 14 | z << 27, blocks in raster order of the reference-block grid
+ * (cpu_source/source.cpp:787-791). nblocks via bm3d_num_blocks(). */
+int bm3d_match(bm3d_plan *plan, unsigned int *d_groups, const bm3d_clip *src, const bm3d_clip *ref,
+               int first_frame, int count, void *stream);
+long long bm3d_num_blocks(const bm3d_plan *plan);
+
+/* Kernel timing (no counterpart in the reference). While enabled, every
+ * launch of the block-matching (kernel 0) and filtering (kernel 1) kernels
+ * is bracketed by HIP events on its stream. bm3d_profile_read waits for the
+ * recorded events and returns the summed milliseconds and launch count of
+ * one kernel since the last reset. */
+int bm3d_profile_enable(int on);
+int bm3d_profile_reset(void);
+int bm3d_profile_read(int kernel, double *total_ms, long long *launches);
+
+/* Exact number of block-matching candidates evaluated (inside the plane)
+ * for the given frames, counted by the matching kernel itself. Used to
+ * state algorithmic work; no counterpart in the reference. */
+int bm3d_count_candidates(bm3d_plan *plan, const bm3d_clip *src, const bm3d_clip *ref, int first_frame,
+                          int count, unsigned long long *ncand, void *stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *bm3d_last_error(void);
+
+/* Library version string. */
+const char *bm3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM3D_MI355X_H */

@@ -1,0 +1,164 @@
+"""Plan-level Python wrapper over the C ABI, for device-resident torch tensors.
+
+A clip tensor is float32 on a HIP device, shaped [frames, planes, height,
+width] and contiguous (row stride = width). torch only provides the device
+memory and the stream; all work runs in libbm3d_mi355x.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _capi as C
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _check_clip(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_cuda:
+        raise TypeError(f"{name} must be a float32 HIP tensor")
+    if t.dim() != 4 or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous [frames, planes, height, width] tensor")
+
+
+def clip_desc(t):
+    n, p, h, w = t.shape
+    return C.Clip(ctypes.c_void_p(t.data_ptr()), p * h * w, h * w, n)
+
+
+class Plan:
+    """One plane group's baked parameters (the role of get_graphexec,
+    source/kernel.cu:759-864)."""
+
+    def __init__(self, width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2,
+                 ps_range=4, chroma=False, final=False, extractor_exp=0, precision="fp32",
+                 stride=None):
+        sig = list(sigma) if isinstance(sigma, (list, tuple)) else [sigma]
+        sig = (sig + [sig[-1]] * 3)[:3]
+        d = C.Desc()
+        d.width, d.height = int(width), int(height)
+        d.stride = int(stride if stride is not None else width)
+        for i in range(3):
+            d.sigma[i] = float(sig[i])
+        d.block_step, d.bm_range, d.radius = int(block_step), int(bm_range), int(radius)
+        d.ps_num, d.ps_range = int(ps_num), int(ps_range)
+        d.chroma, d.final_, d.extractor_exp = int(bool(chroma)), int(bool(final)), int(extractor_exp)
+        d.precision = {"fp32": C.PRECISION_FP32, "mixed": C.PRECISION_MIXED}[precision]
+        self.desc = d
+        self.nplanes = 3 if chroma else 1
+        self.radius = int(radius)
+        self.final = bool(final)
+        h = ctypes.c_void_p()
+        C.check(C.lib().bm3d_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            C.lib().bm3d_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stack_size(self):
+        return int(C.lib().bm3d_stack_size(self._h))
+
+    @property
+    def num_blocks(self):
+        return int(C.lib().bm3d_num_blocks(self._h))
+
+    def new_stacks(self, count, device=None):
+        d = self.desc
+        return torch.empty((count, self.nplanes, 2 * self.radius + 1, 2, d.height, d.stride),
+                           dtype=torch.float32, device=device or "cuda")
+
+    def execute(self, src, ref=None, first=0, count=None, out=None, stream=None):
+        """Stacks [count, planes, 2r+1, 2, H, W] for output frames first.. of src."""
+        _check_clip(src, "clip")
+        if self.final:
+            if ref is None:
+                raise ValueError("final pass needs ref")
+            _check_clip(ref, "ref")
+        n = src.shape[0]
+        count = n - first if count is None else count
+        if out is None:
+            out = self.new_stacks(count, src.device)
+        s = clip_desc(src)
+        r = clip_desc(ref) if ref is not None else None
+        C.check(C.lib().bm3d_execute(self._h, ctypes.c_void_p(out.data_ptr()), 0, 0,
+                                     ctypes.byref(s), ctypes.byref(r) if r else None,
+                                     int(first), int(count), _stream_ptr(stream)))
+        return out
+
+    def count_candidates(self, src, ref=None, first=0, count=None, stream=None):
+        """Exact number of in-plane block-matching candidates for the frames."""
+        _check_clip(src, "clip")
+        n = src.shape[0]
+        count = n - first if count is None else count
+        s = clip_desc(src)
+        r = clip_desc(ref) if ref is not None else None
+        out = ctypes.c_ulonglong()
+        C.check(C.lib().bm3d_count_candidates(self._h, ctypes.byref(s), ctypes.byref(r) if r else None,
+                                              int(first), int(count), ctypes.byref(out),
+                                              _stream_ptr(stream)))
+        return int(out.value)
+
+    def match(self, src, ref=None, first=0, count=None, stream=None):
+        """Packed group lists [count, nblocks, 8] (int32 view of x | y<<14 | z<<27)."""
+        _check_clip(src, "clip")
+        n = src.shape[0]
+        count = n - first if count is None else count
+        g = torch.empty((count, self.num_blocks, 8), dtype=torch.int32, device=src.device)
+        s = clip_desc(src)
+        r = clip_desc(ref) if ref is not None else None
+        C.check(C.lib().bm3d_match(self._h, ctypes.c_void_p(g.data_ptr()), ctypes.byref(s),
+                                   ctypes.byref(r) if r else None, int(first), int(count),
+                                   _stream_ptr(stream)))
+        return g
+
+
+def unpack_groups(g):
+    """[.., 8] packed -> [.., 8, 3] (x, y, z) int64 numpy."""
+    import numpy as np
+    a = np.asarray(g.cpu() if isinstance(g, torch.Tensor) else g).astype(np.int64) & 0xFFFFFFFF
+    x = a & ((1 << 14) - 1)
+    y = (a >> 14) & ((1 << 13) - 1)
+    z = a >> 27
+    return np.stack([x, y, z], axis=-1)
+
+
+def aggregate(stacks, out=None, stream=None):
+    """r = 0: out[f, p] = num / den for stacks [count, planes, 1, 2, H, W]."""
+    cnt, npl, tw, two, h, w = stacks.shape
+    assert tw == 1 and two == 2
+    if out is None:
+        out = torch.empty((cnt, npl, h, w), dtype=torch.float32, device=stacks.device)
+    C.check(C.lib().bm3d_aggregate(ctypes.c_void_p(out.data_ptr()), npl * h * w, h * w, w,
+                                   ctypes.c_void_p(stacks.data_ptr()), 0, 0, w, h, w, npl, cnt,
+                                   _stream_ptr(stream)))
+    return out
+
+
+def vaggregate(stacks, radius, num_frames, first=0, count=None, stack_first=0, out=None,
+               stream=None):
+    """VAggregate over stacks [m, planes, 2r+1, 2, H, W] of frames stack_first..;
+    returns [count, planes, H, W] for output frames first..first+count-1."""
+    m, npl, tw, two, h, w = stacks.shape
+    assert tw == 2 * radius + 1 and two == 2
+    count = num_frames - first if count is None else count
+    if out is None:
+        out = torch.empty((count, npl, h, w), dtype=torch.float32, device=stacks.device)
+    C.check(C.lib().bm3d_vaggregate(ctypes.c_void_p(out.data_ptr()), npl * h * w, h * w, w,
+                                    ctypes.c_void_p(stacks.data_ptr()), 0, 0, int(stack_first),
+                                    w, h, w, npl, int(radius), int(num_frames), int(first),
+                                    int(count), _stream_ptr(stream)))
+    return out

@@ -1,0 +1,476 @@
+// Launch layer: the extern "C" entry points of include/bm3d_mi355x.h.
+//
+// Replaces get_graphexec (source/kernel.cu:759-864): instead of a graph with
+// baked HtoD / memset / kernel / DtoH nodes per frame, a plan holds the
+// parameters and the device scratch (group lists, corner-weight maps), and
+// bm3d_execute runs zero -> match -> filter -> den-box on device-resident
+// clips for a whole batch of output frames per launch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bm3d_kernels.h"
+#include "bm3d_mi355x.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(BM3D_EHIP, std::string("BM3D: '") + what + "' failed: " + hipGetErrorString(e));
+}
+
+#define BM3D_HIP(expr)                                         \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);      \
+    } while (0)
+
+// Frames per launch: bounds the scratch (group lists and corner maps).
+constexpr int kMaxBatchFrames = 16;
+
+// Optional per-kernel timing with HIP events (bm3d_profile_*).
+struct Profiler {
+    std::mutex mu;
+    bool on = false;
+    struct Rec { int kernel; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+};
+Profiler& profiler() {
+    static Profiler p;
+    return p;
+}
+
+// Brackets one launch with events when profiling is on.
+struct ProfScope {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kernel;
+    hipStream_t s;
+    ProfScope(int k, hipStream_t st) : kernel(k), s(st) {
+        Profiler& P = profiler();
+        std::lock_guard<std::mutex> g(P.mu);
+        if (!P.on) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+        (void)hipEventRecord(a, s);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        (void)hipEventRecord(b, s);
+        Profiler& P = profiler();
+        std::lock_guard<std::mutex> g(P.mu);
+        P.recs.push_back({kernel, a, b});
+    }
+};
+
+}  // namespace
+
+struct bm3d_plan {
+    bm3d_desc d{};
+    float sigma[3]{};   // scaled thresholds
+    float extractor = 0.0f;
+    int nplanes = 1, tw = 1;
+    int nbx = 0, nby = 0, nblocks = 0;
+    int margin = 0, tby = 16, tile_w = 0, tile_h = 0, tile_wq = 0, tile_hp = 0;
+    size_t lds_bytes = 0;
+    long long plane_size = 0, stack_size = 0, corner_frame = 0;
+    int device = 0;
+    uint32_t* groups = nullptr;
+    float* corner = nullptr;
+    int cap_frames = 0;
+};
+
+namespace {
+
+int count_positions(int n, int bs) {
+    int c = 0;
+    for (int v = 0; v < n - 8 + bs; v += bs) ++c;
+    return c;
+}
+
+// Tile geometry for a work-group of 16 x tby reference blocks.
+void tile_geometry(bm3d_plan* p, int tby) {
+    const int bs = p->d.block_step;
+    p->tby = tby;
+    p->tile_w = 15 * bs + 8 + 2 * p->margin;
+    p->tile_h = (tby - 1) * bs + 8 + 2 * p->margin;
+    p->tile_wq = (p->tile_w + bs - 1) / bs;
+    p->tile_hp = p->tile_h | 1;  // odd column pitch: lanes of a row hit distinct banks
+    p->lds_bytes = (size_t)bs * p->tile_wq * p->tile_hp * sizeof(float);
+}
+
+int ensure_scratch(bm3d_plan* p, int frames) {
+    if (frames <= p->cap_frames) return BM3D_OK;
+    if (p->groups) (void)hipFree(p->groups);
+    if (p->corner) (void)hipFree(p->corner);
+    p->groups = nullptr;
+    p->corner = nullptr;
+    p->cap_frames = 0;
+    BM3D_HIP(hipMalloc(&p->groups, (size_t)frames * p->nblocks * 8 * sizeof(uint32_t)));
+    BM3D_HIP(hipMalloc(&p->corner, (size_t)frames * p->corner_frame * sizeof(float)));
+    p->cap_frames = frames;
+    return BM3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bm3d_last_error(void) { return g_last_error.c_str(); }
+
+const char* bm3d_version(void) { return "bm3d_mi355x 0.1 (gfx950)"; }
+
+int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(BM3D_EINVAL, "BM3D: null argument");
+    *out = nullptr;
+    const bm3d_desc& d = *desc;
+    // argument checks, same order and wording as source/source.cpp:503-611
+    for (int i = 0; i < 3; ++i)
+        if (d.sigma[i] < 0.0f) return fail(BM3D_EINVAL, "BM3D: \"sigma\" must be non-negative");
+    if (d.block_step <= 0 || d.block_step > 8)
+        return fail(BM3D_EINVAL, "BM3D: \"block_step\" must be in range [1, 8]");
+    if (d.bm_range <= 0) return fail(BM3D_EINVAL, "BM3D: \"bm_range\" must be positive");
+    if (d.radius < 0) return fail(BM3D_EINVAL, "BM3D: \"radius\" must be non-negative");
+    if (d.ps_num <= 0 || d.ps_num > 8) return fail(BM3D_EINVAL, "BM3D: \"ps_num\" must be in range [1, 8]");
+    if (d.ps_range <= 0) return fail(BM3D_EINVAL, "BM3D: \"ps_range\" must be positive");
+    // limits the reference leaves unchecked
+    if (d.width < 8 || d.height < 8) return fail(BM3D_EINVAL, "BM3D: plane must be at least 8x8");
+    if (d.stride < d.width) return fail(BM3D_EINVAL, "BM3D: stride must be >= width");
+    if (d.width > bm3d::kMaxWidth || d.height > bm3d::kMaxHeight)
+        return fail(BM3D_EUNSUPPORTED, "BM3D: plane larger than 16384x8192 is not supported");
+    if (d.radius > bm3d::kMaxRadius) return fail(BM3D_EUNSUPPORTED, "BM3D: \"radius\" above 15 is not supported");
+    if (d.extractor_exp < 0 || d.extractor_exp > 100)
+        return fail(BM3D_EINVAL, "BM3D: \"extractor_exp\" out of range");
+    if (d.precision != BM3D_PRECISION_FP32 && d.precision != BM3D_PRECISION_MIXED)
+        return fail(BM3D_EINVAL, "BM3D: unknown precision");
+    if (d.precision == BM3D_PRECISION_MIXED)
+        return fail(BM3D_EUNSUPPORTED, "BM3D: mixed precision is not available in this build");
+
+    auto* p = new bm3d_plan;
+    p->d = d;
+    p->d.chroma = d.chroma ? 1 : 0;
+    p->d.final_ = d.final_ ? 1 : 0;
+    // sigma scaling, source/source.cpp:520-523 (float, left to right)
+    const float f = (3.0f / 4.0f) / 255.0f * 64.0f * (p->d.final_ ? 1.0f : 2.7f);
+    for (int i = 0; i < 3; ++i) p->sigma[i] = d.sigma[i] * f;
+    // extractor = 2^e, source/source.cpp:623-629
+    p->extractor = d.extractor_exp ? std::ldexp(1.0f, d.extractor_exp) : 0.0f;
+    p->nplanes = p->d.chroma ? 3 : 1;
+    p->tw = 2 * d.radius + 1;
+    p->nbx = count_positions(d.width, d.block_step);
+    p->nby = count_positions(d.height, d.block_step);
+    p->nblocks = p->nbx * p->nby;
+    p->margin = d.bm_range + d.radius * d.ps_range;
+    p->plane_size = (long long)d.height * d.stride;
+    p->stack_size = (long long)p->nplanes * p->tw * 2 * p->plane_size;
+    p->corner_frame = (long long)p->nplanes * p->tw * p->plane_size;
+
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(e, "hipGetDevice");
+    }
+    int max_lds = 0;
+    e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(e, "hipDeviceGetAttribute");
+    }
+    tile_geometry(p, 16);
+    if (p->lds_bytes > 65536) tile_geometry(p, 4);
+    if (p->lds_bytes > (size_t)max_lds) {
+        delete p;
+        return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
+    }
+    if (p->lds_bytes > 65536) {
+        e = bm3d::set_match_lds_limit(p->lds_bytes);
+        if (e != hipSuccess) {
+            delete p;
+            return hip_fail(e, "hipFuncSetAttribute");
+        }
+    }
+    *out = p;
+    return BM3D_OK;
+}
+
+void bm3d_plan_destroy(bm3d_plan* p) {
+    if (!p) return;
+    if (p->groups) (void)hipFree(p->groups);
+    if (p->corner) (void)hipFree(p->corner);
+    delete p;
+}
+
+long long bm3d_stack_size(const bm3d_plan* p) { return p ? p->stack_size : 0; }
+
+int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long long res_plane_stride,
+                 const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count, void* stream_) {
+    g_last_error.clear();
+    if (!p || !d_res || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (p->d.final_ && (ref->frame_stride != src->frame_stride || ref->plane_stride != src->plane_stride ||
+                        ref->num_frames != src->num_frames))
+        return fail(BM3D_EINVAL, "BM3D: \"ref\" must have the layout and number of frames of \"clip\"");
+    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    const long long plane_stack = (long long)p->tw * 2 * p->plane_size;
+    if (res_plane_stride == 0) res_plane_stride = plane_stack;
+    if (res_frame_stride == 0) res_frame_stride = p->nplanes * res_plane_stride;
+    if (res_plane_stride < plane_stack || res_frame_stride < (p->nplanes - 1) * res_plane_stride + plane_stack)
+        return fail(BM3D_EINVAL, "BM3D: result strides overlap");
+    if (count == 0) return BM3D_OK;
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    int dev = 0;
+    BM3D_HIP(hipGetDevice(&dev));
+    if (dev != p->device) return fail(BM3D_EINVAL, "BM3D: plan used on a different device than it was created on");
+
+    const int batch = std::min(count, kMaxBatchFrames);
+    if (int rc = ensure_scratch(p, batch)) return rc;
+
+    const bm3d_desc& d = p->d;
+    for (int f0 = 0; f0 < count; f0 += batch) {
+        const int nf = std::min(batch, count - f0);
+        const int n0 = first_frame + f0;
+        float* res = d_res + (size_t)f0 * res_frame_stride;
+        if (res_plane_stride == plane_stack) {
+            BM3D_HIP(bm3d::launch_zero(res, (size_t)p->nplanes * plane_stack, (size_t)res_frame_stride, nf, stream));
+        } else {
+            for (int f = 0; f < nf; ++f)
+                BM3D_HIP(bm3d::launch_zero(res + (size_t)f * res_frame_stride, (size_t)plane_stack,
+                                           (size_t)res_plane_stride, p->nplanes, stream));
+        }
+        BM3D_HIP(bm3d::launch_zero(p->corner, (size_t)nf * p->corner_frame, 0, 1, stream));
+
+        bm3d::MatchArgs m{};
+        m.clip = d.final_ ? ref->data : src->data;  // matching runs on ref in the final pass
+        m.frame_stride = src->frame_stride;
+        m.num_frames = src->num_frames;
+        m.first_frame = n0;
+        m.W = d.width; m.H = d.height; m.stride = d.stride;
+        m.bs = d.block_step; m.bm_range = d.bm_range; m.radius = d.radius;
+        m.ps_num = d.ps_num; m.ps_range = d.ps_range;
+        m.nbx = p->nbx; m.nby = p->nby; m.nblocks = p->nblocks;
+        m.margin = p->margin;
+        m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
+        m.groups = p->groups;
+        {
+            ProfScope ps(0, stream);
+            BM3D_HIP(bm3d::launch_match(m, p->tby, p->lds_bytes, nf, stream));
+        }
+
+        bm3d::FilterArgs fa{};
+        fa.src = src->data;
+        fa.ref = d.final_ ? ref->data : nullptr;
+        fa.frame_stride = src->frame_stride;
+        fa.plane_stride = src->plane_stride;
+        fa.num_frames = src->num_frames;
+        fa.first_frame = n0;
+        fa.W = d.width; fa.H = d.height; fa.stride = d.stride;
+        fa.plane_size = p->plane_size;
+        fa.radius = d.radius;
+        fa.nblocks = p->nblocks;
+        for (int i = 0; i < 3; ++i) fa.sigma[i] = p->sigma[i];
+        fa.extractor = p->extractor;
+        fa.groups = p->groups;
+        fa.res = res;
+        fa.res_frame_stride = res_frame_stride;
+        fa.res_plane_stride = res_plane_stride;
+        fa.corner = p->corner;
+        fa.corner_frame_stride = p->corner_frame;
+        {
+            ProfScope ps(1, stream);
+            BM3D_HIP(bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, stream));
+        }
+
+        bm3d::DenArgs da{};
+        da.corner = p->corner;
+        da.corner_frame_stride = p->corner_frame;
+        da.res = res;
+        da.res_frame_stride = res_frame_stride;
+        da.res_plane_stride = res_plane_stride;
+        da.W = d.width; da.H = d.height; da.stride = d.stride;
+        da.plane_size = p->plane_size;
+        da.nplanes = p->nplanes;
+        da.slots = p->tw;
+        BM3D_HIP(bm3d::launch_den_box(da, nf * p->nplanes * p->tw, stream));
+    }
+    return BM3D_OK;
+}
+
+long long bm3d_num_blocks(const bm3d_plan* p) { return p ? p->nblocks : 0; }
+
+int bm3d_profile_enable(int on) {
+    Profiler& P = profiler();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.on = on != 0;
+    return BM3D_OK;
+}
+
+int bm3d_profile_reset(void) {
+    Profiler& P = profiler();
+    std::lock_guard<std::mutex> g(P.mu);
+    for (auto& r : P.recs) {
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    P.recs.clear();
+    return BM3D_OK;
+}
+
+int bm3d_profile_read(int kernel, double* total_ms, long long* launches) {
+    g_last_error.clear();
+    if (!total_ms || !launches) return fail(BM3D_EINVAL, "BM3D: null argument");
+    Profiler& P = profiler();
+    std::lock_guard<std::mutex> g(P.mu);
+    double t = 0.0;
+    long long n = 0;
+    for (auto& r : P.recs) {
+        if (r.kernel != kernel) continue;
+        BM3D_HIP(hipEventSynchronize(r.b));
+        float ms = 0.0f;
+        BM3D_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        ++n;
+    }
+    *total_ms = t;
+    *launches = n;
+    return BM3D_OK;
+}
+
+int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
+                          unsigned long long* ncand, void* stream) {
+    g_last_error.clear();
+    if (!p || !src || !src->data || !ncand) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    unsigned long long* d_cnt = nullptr;
+    BM3D_HIP(hipMalloc(&d_cnt, sizeof(unsigned long long)));
+    const bm3d_desc& d = p->d;
+    bm3d::MatchArgs m{};
+    m.clip = d.final_ ? ref->data : src->data;
+    m.frame_stride = src->frame_stride;
+    m.num_frames = src->num_frames;
+    m.first_frame = first_frame;
+    m.W = d.width; m.H = d.height; m.stride = d.stride;
+    m.bs = d.block_step; m.bm_range = d.bm_range; m.radius = d.radius;
+    m.ps_num = d.ps_num; m.ps_range = d.ps_range;
+    m.nbx = p->nbx; m.nby = p->nby; m.nblocks = p->nblocks;
+    m.margin = p->margin;
+    m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
+    m.groups = nullptr;
+    m.ncand = d_cnt;
+    hipError_t e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess && count > 0) e = bm3d::launch_match(m, p->tby, p->lds_bytes, count, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ncand, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_cnt);
+    if (e != hipSuccess) return hip_fail(e, "bm3d_count_candidates");
+    return BM3D_OK;
+}
+
+int bm3d_match(bm3d_plan* p, unsigned int* d_groups, const bm3d_clip* src, const bm3d_clip* ref,
+               int first_frame, int count, void* stream) {
+    g_last_error.clear();
+    if (!p || !d_groups || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    if (count == 0) return BM3D_OK;
+    const bm3d_desc& d = p->d;
+    bm3d::MatchArgs m{};
+    m.clip = d.final_ ? ref->data : src->data;
+    m.frame_stride = src->frame_stride;
+    m.num_frames = src->num_frames;
+    m.first_frame = first_frame;
+    m.W = d.width; m.H = d.height; m.stride = d.stride;
+    m.bs = d.block_step; m.bm_range = d.bm_range; m.radius = d.radius;
+    m.ps_num = d.ps_num; m.ps_range = d.ps_range;
+    m.nbx = p->nbx; m.nby = p->nby; m.nblocks = p->nblocks;
+    m.margin = p->margin;
+    m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
+    m.groups = d_groups;
+    BM3D_HIP(bm3d::launch_match(m, p->tby, p->lds_bytes, count, reinterpret_cast<hipStream_t>(stream)));
+    return BM3D_OK;
+}
+
+int bm3d_execute_window(bm3d_plan* p, float* d_res, const float* d_src, void* stream) {
+    if (!p || !d_res || !d_src) return fail(BM3D_EINVAL, "BM3D: null argument");
+    const long long fs = p->plane_size;           // one slot
+    const long long ps = (long long)p->tw * fs;   // one plane of slots
+    const long long clip_stride = p->nplanes * ps;
+    bm3d_clip ref{d_src, fs, ps, p->tw};
+    bm3d_clip src{d_src + (p->d.final_ ? clip_stride : 0), fs, ps, p->tw};
+    return bm3d_execute(p, d_res, 0, 0, &src, p->d.final_ ? &ref : nullptr, p->d.radius, 1, stream);
+}
+
+int bm3d_aggregate(float* dst, long long dst_frame_stride, long long dst_plane_stride, int dst_stride,
+                   const float* res, long long res_frame_stride, long long res_plane_stride, int width,
+                   int height, int stride, int nplanes, int count, void* stream) {
+    g_last_error.clear();
+    if (!dst || !res) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (width <= 0 || height <= 0 || stride < width || dst_stride < width || nplanes <= 0 || count < 0)
+        return fail(BM3D_EINVAL, "BM3D: invalid aggregation shape");
+    if (count == 0) return BM3D_OK;
+    bm3d::DivideArgs a{};
+    a.res = res;
+    a.plane_size = (long long)height * stride;
+    a.res_plane_stride = res_plane_stride ? res_plane_stride : 2 * a.plane_size;
+    a.res_frame_stride = res_frame_stride ? res_frame_stride : nplanes * a.res_plane_stride;
+    a.dst = dst;
+    a.dst_frame_stride = dst_frame_stride;
+    a.dst_plane_stride = dst_plane_stride;
+    a.dst_stride = dst_stride;
+    a.W = width; a.H = height; a.stride = stride;
+    a.nplanes = nplanes;
+    BM3D_HIP(bm3d::launch_divide(a, count * nplanes, reinterpret_cast<hipStream_t>(stream)));
+    return BM3D_OK;
+}
+
+int bm3d_vaggregate(float* dst, long long dst_frame_stride, long long dst_plane_stride, int dst_stride,
+                    const float* stacks, long long stack_frame_stride, long long stack_plane_stride,
+                    int stack_first, int width, int height, int stride, int nplanes, int radius,
+                    int num_frames, int first_frame, int count, void* stream) {
+    g_last_error.clear();
+    if (!dst || !stacks) return fail(BM3D_EINVAL, "VAggregate: null argument");
+    if (width <= 0 || height <= 0 || stride < width || dst_stride < width || nplanes <= 0 || radius < 0 ||
+        count < 0 || first_frame < 0 || first_frame + count > num_frames)
+        return fail(BM3D_EINVAL, "VAggregate: invalid shape or frame range");
+    if (count == 0) return BM3D_OK;
+    // the stacks must cover every frame the window touches
+    if (stack_first > std::max(first_frame - radius, 0))
+        return fail(BM3D_EINVAL, "VAggregate: stacks do not cover the temporal window");
+    bm3d::VAggArgs a{};
+    a.stacks = stacks;
+    a.plane_size = (long long)height * stride;
+    a.stack_plane_stride = stack_plane_stride ? stack_plane_stride : (long long)(2 * radius + 1) * 2 * a.plane_size;
+    a.stack_frame_stride = stack_frame_stride ? stack_frame_stride : nplanes * a.stack_plane_stride;
+    a.stack_first = stack_first;
+    a.dst = dst;
+    a.dst_frame_stride = dst_frame_stride;
+    a.dst_plane_stride = dst_plane_stride;
+    a.dst_stride = dst_stride;
+    a.W = width; a.H = height; a.stride = stride;
+    a.nplanes = nplanes;
+    a.radius = radius;
+    a.num_frames = num_frames;
+    a.first_frame = first_frame;
+    BM3D_HIP(bm3d::launch_vaggregate(a, count * nplanes, reinterpret_cast<hipStream_t>(stream)));
+    return BM3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Kernel argument blocks and launchers shared by bm3d_kernels.hip and the
+// launch layer (bm3d_capi.cpp). Internal: not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bm3d {
+
+// Packed block position: x in bits 0-13, y in bits 14-26, slot z in 27-31.
+constexpr int kPosYShift = 14;
+constexpr int kPosZShift = 27;
+constexpr uint32_t kPosXMask = (1u << 14) - 1;
+constexpr uint32_t kPosYMask = (1u << 13) - 1;
+constexpr int kMaxWidth = 1 << 14;
+constexpr int kMaxHeight = 1 << 13;
+constexpr int kMaxRadius = 15;
+
+// Floats per group in the filter kernel's transpose buffer (512 + padding
+// that spreads the 8 groups of a wave over different LDS banks).
+constexpr int kTransposeGroupStride = 520;
+
+struct MatchArgs {
+    const float* clip;       // plane used for matching, frame 0
+    long long frame_stride;  // floats between frames
+    int num_frames;          // frames in the clip (slot indices clamp to it)
+    int first_frame;         // output frame of grid z = 0
+    int W, H, stride;
+    int bs, bm_range, radius, ps_num, ps_range;
+    int nbx, nby, nblocks;
+    int margin;              // bm_range + radius * ps_range
+    int tile_w, tile_h, tile_wq, tile_hp;
+    uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
+    unsigned long long* ncand;  // non-null: count in-plane candidates
+};
+
+struct FilterArgs {
+    const float* src;        // noisy clip, frame 0 plane 0
+    const float* ref;        // basic-estimate clip (final pass), same layout
+    long long frame_stride, plane_stride;
+    int num_frames, first_frame;
+    int W, H, stride;
+    long long plane_size;    // H * stride
+    int radius;
+    int nblocks;
+    float sigma[3];          // scaled thresholds per plane
+    float extractor;         // 2^extractor_exp or 0
+    const uint32_t* groups;  // [frames][nblocks][8]
+    float* res;              // per frame: [planes][2r+1][2][H][stride]
+    long long res_frame_stride, res_plane_stride;
+    float* corner;           // [frames][planes][2r+1][H][stride]
+    long long corner_frame_stride;
+};
+
+struct DenArgs {
+    const float* corner;
+    long long corner_frame_stride;
+    float* res;
+    long long res_frame_stride, res_plane_stride;
+    int W, H, stride;
+    long long plane_size;
+    int nplanes, slots;
+};
+
+struct DivideArgs {
+    const float* res;        // r = 0 stacks [frames][planes][1][2][H][stride]
+    long long res_frame_stride, res_plane_stride;
+    float* dst;
+    long long dst_frame_stride, dst_plane_stride;
+    int dst_stride;
+    int W, H, stride;
+    long long plane_size;
+    int nplanes;
+};
+
+struct VAggArgs {
+    const float* stacks;     // stacks of frames [stack_first, ...)
+    long long stack_frame_stride, stack_plane_stride;
+    int stack_first;
+    float* dst;              // output frames [first_frame, first_frame + count)
+    long long dst_frame_stride, dst_plane_stride;
+    int dst_stride;
+    int W, H, stride;
+    long long plane_size;
+    int nplanes;             // planes in each stack (processed planes only)
+    int radius, num_frames, first_frame;
+};
+
+hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
+hipError_t set_match_lds_limit(size_t bytes);
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, hipStream_t s);
+// Zeroes `count` runs of `n` floats, `step` floats apart.
+hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s);
+hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s);
+hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s);
+hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s);
+
+}  // namespace bm3d

@@ -1,0 +1,706 @@
+// gfx950 (MI355X) kernels for BM3D / V-BM3D.
+//
+// The reference runs the whole per-block pipeline in one CUDA kernel with a
+// 32-thread block per 4 reference blocks (source/kernel.cu:366-757). Here the
+// pipeline is split by resource profile:
+//
+//   bm3d_match     one lane per reference block. The work-group stages the
+//                  union search window of its 16 x TBY blocks in LDS, split
+//                  into block_step column phases so that the 16 lanes of a
+//                  row read consecutive words. Each lane slides an 8-column
+//                  register ring across the window (8 LDS reads per
+//                  candidate) and keeps its top-8 in registers.
+//   bm3d_filter    8 lanes per group, 8 groups per wave: gather, 3-D DCT,
+//                  hard threshold or Wiener shrinkage, inverse, and
+//                  aggregation with float atomics (pre-rounded when
+//                  extractor_exp is set). Per-member weights are added once
+//                  at the member's corner; bm3d_den_box turns the corner map
+//                  into the per-pixel weight rows with an 8x8 box sum.
+//   bm3d_zero / bm3d_divide / bm3d_vaggregate   elementwise passes.
+//
+// Every float operation is written out (the build uses -ffp-contract=off),
+// so results match the CPU oracle bit for bit up to the order of the
+// aggregation atomics.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bm3d_kernels.h"
+
+namespace bm3d {
+
+namespace {
+
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kFltEpsilon = 1.192092896e-07f;
+
+// Cosine constants of the 8-point DCT, written as double literals rounded
+// once to float (the oracle rounds the same literals the same way).
+constexpr float kR2 = static_cast<float>(1.41421356237309504880168872420969808);
+constexpr float kK1 = static_cast<float>(1.84775906502257351225636637879357657);  // 2 cos(pi/8)
+constexpr float kK3 = static_cast<float>(0.76536686473017954345691996806916186);  // 2 cos(3pi/8)
+constexpr float kC1 = static_cast<float>(1.96157056080646086901156744484086917);  // 2 cos(pi/16)
+constexpr float kC3 = static_cast<float>(1.66293922460509004760575233432468773);  // 2 cos(3pi/16)
+constexpr float kC5 = static_cast<float>(1.11114046603920391150735136234616736);  // 2 cos(5pi/16)
+constexpr float kC7 = static_cast<float>(0.39018064403225654488047858302562963);  // 2 cos(7pi/16)
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// 8-point DCT-II scaled to 4 x orthonormal (the reference's normalisation,
+// cpu_source/source.cpp:444-536), by an even/odd split: a = x + rev(x) feeds
+// the even outputs through a 4-point DCT, b = x - rev(x) the odd outputs
+// through a symmetric 4x4 cosine matrix.
+__device__ __forceinline__ void dct8_fwd(float& v0, float& v1, float& v2, float& v3,
+                                         float& v4, float& v5, float& v6, float& v7) {
+    float a0 = v0 + v7, a1 = v1 + v6, a2 = v2 + v5, a3 = v3 + v4;
+    float b0 = v0 - v7, b1 = v1 - v6, b2 = v2 - v5, b3 = v3 - v4;
+    float c0 = a0 + a3, c1 = a1 + a2, d0 = a0 - a3, d1 = a1 - a2;
+    v0 = kR2 * (c0 + c1);
+    v4 = kR2 * (c0 - c1);
+    v2 = fma_(kK1, d0, kK3 * d1);
+    v6 = fma_(kK3, d0, -(kK1 * d1));
+    v1 = fma_(kC1, b0, fma_(kC3, b1, fma_(kC5, b2, kC7 * b3)));
+    v3 = fma_(kC3, b0, fma_(-kC7, b1, fma_(-kC1, b2, -(kC5 * b3))));
+    v5 = fma_(kC5, b0, fma_(-kC1, b1, fma_(kC7, b2, kC3 * b3)));
+    v7 = fma_(kC7, b0, fma_(-kC5, b1, fma_(kC3, b2, -(kC1 * b3))));
+}
+
+// Transpose of dct8_fwd (DCT-III, 4 x orthonormal): dct8_inv(dct8_fwd(x)) = 16 x.
+__device__ __forceinline__ void dct8_inv(float& v0, float& v1, float& v2, float& v3,
+                                         float& v4, float& v5, float& v6, float& v7) {
+    float gc0 = kR2 * (v0 + v4), gc1 = kR2 * (v0 - v4);
+    float gd0 = fma_(kK1, v2, kK3 * v6);
+    float gd1 = fma_(kK3, v2, -(kK1 * v6));
+    float ga0 = gc0 + gd0, ga3 = gc0 - gd0, ga1 = gc1 + gd1, ga2 = gc1 - gd1;
+    float gb0 = fma_(kC1, v1, fma_(kC3, v3, fma_(kC5, v5, kC7 * v7)));
+    float gb1 = fma_(kC3, v1, fma_(-kC7, v3, fma_(-kC1, v5, -(kC5 * v7))));
+    float gb2 = fma_(kC5, v1, fma_(-kC1, v3, fma_(kC7, v5, kC3 * v7)));
+    float gb3 = fma_(kC7, v1, fma_(-kC5, v3, fma_(kC3, v5, -(kC1 * v7))));
+    v0 = ga0 + gb0; v7 = ga0 - gb0;
+    v1 = ga1 + gb1; v6 = ga1 - gb1;
+    v2 = ga2 + gb2; v5 = ga2 - gb2;
+    v3 = ga3 + gb3; v4 = ga3 - gb3;
+}
+
+template <bool kInverse>
+__device__ __forceinline__ void dct8(float* v) {
+    if constexpr (kInverse) dct8_inv(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+    else dct8_fwd(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+}
+
+// Along the first index of v[8][8] (v[i][c] for fixed c).
+template <bool kInverse>
+__device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
+    if constexpr (kInverse)
+        dct8_inv(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c], v[5][c], v[6][c], v[7][c]);
+    else
+        dct8_fwd(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c], v[5][c], v[6][c], v[7][c]);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ uint32_t pack_pos(int x, int y, int z) {
+    return (uint32_t)x | ((uint32_t)y << kPosYShift) | ((uint32_t)z << kPosZShift);
+}
+__device__ __forceinline__ int pos_x(uint32_t p) { return (int)(p & kPosXMask); }
+__device__ __forceinline__ int pos_y(uint32_t p) { return (int)((p >> kPosYShift) & kPosYMask); }
+__device__ __forceinline__ int pos_z(uint32_t p) { return (int)(p >> kPosZShift); }
+
+// ---------------------------------------------------------------- matching
+
+// Sorted (ascending, strict <) list of the 8 best candidates, kept in
+// registers. Equivalent to the shuffle-up insertion of source/kernel.cu:473-485
+// and the SIMD insertion of cpu_source/source.cpp:179-204.
+struct Best8 {
+    float e[8];
+    uint32_t p[8];
+
+    __device__ __forceinline__ void init(float err, uint32_t pos) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { e[i] = err; p[i] = pos; }
+    }
+
+    __device__ __forceinline__ void insert(float err, uint32_t pos) {
+        bool c[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) c[i] = err < e[i];
+#pragma unroll
+        for (int i = 7; i >= 1; --i) {
+            float ne = c[i - 1] ? e[i - 1] : (c[i] ? err : e[i]);
+            uint32_t np = c[i - 1] ? p[i - 1] : (c[i] ? pos : p[i]);
+            e[i] = ne;
+            p[i] = np;
+        }
+        e[0] = c[0] ? err : e[0];
+        p[0] = c[0] ? pos : p[0];
+    }
+
+    // entry i for a wave-uniform runtime i
+    __device__ __forceinline__ float err_at(int i) const {
+        float r = e[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) r = (i == k) ? e[k] : r;
+        return r;
+    }
+    __device__ __forceinline__ uint32_t pos_at(int i) const {
+        uint32_t r = p[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) r = (i == k) ? p[k] : r;
+        return r;
+    }
+};
+
+// LDS tile: image rows [ty0, ty0 + th) x columns [tx0, tx0 + tw), stored as
+// `bs` column phases; phase ph holds columns ph, ph + bs, ... at consecutive
+// column slots, and each column is `hp` consecutive words (rows). Lanes of a
+// work-group row hold blocks bs columns apart, so they read the same phase
+// at consecutive column slots.
+struct TileGeom {
+    int tx0, ty0;  // image coordinates of the tile origin (may be negative)
+    int tw, th;    // tile extent
+    int bs;        // number of column phases
+    int wq;        // column slots per phase
+    int hp;        // words per column (>= th)
+};
+
+__device__ __forceinline__ int tile_col_base(const TileGeom& g, int tc) {
+    int q = tc / g.bs;
+    int ph = tc - q * g.bs;
+    return (ph * g.wq + q) * g.hp;
+}
+
+// Fill the tile from one plane; out-of-image pixels replicate the border
+// (they are only ever read for candidates that are masked off).
+__device__ void stage_tile(float* __restrict__ lds, const TileGeom& g,
+                           const float* __restrict__ img, int W, int H, int stride) {
+    const int n = g.tw * g.th;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+        int r = idx / g.tw;
+        int c = idx - r * g.tw;
+        int gx = clampi(g.tx0 + c, 0, W - 1);
+        int gy = clampi(g.ty0 + r, 0, H - 1);
+        lds[tile_col_base(g, c) + r] = img[(size_t)gy * stride + gx];
+    }
+}
+
+// Load one 8-row column of the tile into a ring slot.
+__device__ __forceinline__ void load_col(float (&dst)[8], const float* __restrict__ src) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dst[j] = src[j];
+}
+
+// Distance in the exact order of cpu_source/source.cpp:123-136 + 98-103:
+// per column an even-row and an odd-row FMA chain, then a pairwise tree.
+// ring slot of candidate column c is (u + c) & 7.
+template <int U>
+__device__ __forceinline__ float ring_ssd(const float (&rb)[8][8], const float (&ring)[8][8]) {
+    float s[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int sl = (U + c) & 7;
+        float d0 = rb[0][c] - ring[sl][0];
+        float d1 = rb[1][c] - ring[sl][1];
+        float d2 = rb[2][c] - ring[sl][2];
+        float d3 = rb[3][c] - ring[sl][3];
+        float d4 = rb[4][c] - ring[sl][4];
+        float d5 = rb[5][c] - ring[sl][5];
+        float d6 = rb[6][c] - ring[sl][6];
+        float d7 = rb[7][c] - ring[sl][7];
+        float e0 = d0 * d0;
+        e0 = fma_(d2, d2, e0);
+        e0 = fma_(d4, d4, e0);
+        e0 = fma_(d6, d6, e0);
+        float e1 = d1 * d1;
+        e1 = fma_(d3, d3, e1);
+        e1 = fma_(d5, d5, e1);
+        e1 = fma_(d7, d7, e1);
+        s[c] = e0 + e1;
+    }
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+template <int U, bool kCount>
+__device__ __forceinline__ void scan_step(const float* __restrict__ lds, const TileGeom& g,
+                                          const float (&rb)[8][8], float (&ring)[8][8],
+                                          int& ph, int& colbase, int rt, Best8& best,
+                                          int col, int row, bool row_ok, int z, int W,
+                                          unsigned& cnt) {
+    load_col(ring[(U + 7) & 7], lds + colbase + rt);
+    // advance to the next tile column
+    ++ph;
+    const bool wrap = ph == g.bs;
+    ph = wrap ? 0 : ph;
+    colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
+    float e = ring_ssd<U>(rb, ring);
+    const bool ok = row_ok && col >= 0 && col <= W - 8;
+    if constexpr (kCount) cnt += ok ? 1u : 0u;
+    if (ok && e < best.e[7]) best.insert(e, pack_pos(col, row, z));
+}
+
+// Exhaustive scan of the window of radius `range` around (cx, cy), clamped
+// to the plane, row-major (cpu_source/source.cpp:143-215). All lanes walk
+// the full (2 range + 1)^2 square so the loop is wave-uniform; candidates
+// outside the plane are masked, which leaves the scan order of the valid
+// ones -- and hence tie-breaking -- unchanged.
+template <bool kCount>
+__device__ __forceinline__ void scan_window(const float* __restrict__ lds, const TileGeom& g,
+                                            const float (&rb)[8][8], Best8& best,
+                                            int cx, int cy, int range, int z, int W, int H,
+                                            unsigned& cnt) {
+    const int n = 2 * range + 1;
+    float ring[8][8];
+    for (int dyi = 0; dyi < n; ++dyi) {
+        const int row = cy - range + dyi;
+        const bool row_ok = row >= 0 && row <= H - 8;
+        const int rt = row - g.ty0;
+        const int tc0 = cx - range - g.tx0;
+        int q = tc0 / g.bs;
+        int ph = tc0 - q * g.bs;
+        int colbase = (ph * g.wq + q) * g.hp;
+        // preload the first 7 columns
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            load_col(ring[s], lds + colbase + rt);
+            ++ph;
+            const bool wrap = ph == g.bs;
+            ph = wrap ? 0 : ph;
+            colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
+        }
+        const int col0 = cx - range;
+        for (int base = 0; base < n; base += 8) {
+#define BM3D_SCAN_STEP(U)                                                                    \
+    if (base + U < n)                                                                        \
+        scan_step<U, kCount>(lds, g, rb, ring, ph, colbase, rt, best, col0 + base + U, row, row_ok, z, W, cnt);
+            BM3D_SCAN_STEP(0)
+            BM3D_SCAN_STEP(1)
+            BM3D_SCAN_STEP(2)
+            BM3D_SCAN_STEP(3)
+            BM3D_SCAN_STEP(4)
+            BM3D_SCAN_STEP(5)
+            BM3D_SCAN_STEP(6)
+            BM3D_SCAN_STEP(7)
+#undef BM3D_SCAN_STEP
+        }
+    }
+}
+
+// Block matching: one lane per reference block; a work-group covers 16 x TBY
+// reference blocks (TBY = blockDim.x / 16). Grid: (ceil(nbx/16),
+// ceil(nby/TBY), frames). Spatial search on the centre slot (source/kernel.cu
+// :432-492), then for r > 0 the predictive temporal search
+// (source/kernel.cu:495-614, cpu_source/source.cpp:220-321), then the centre
+// insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
+// with kCount, also adds the number of in-plane candidates to *a.ncand.
+template <bool kCount>
+__global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
+    extern __shared__ float lds[];
+    const int tby = blockDim.x >> 4;
+    const int lx = threadIdx.x & 15;
+    const int ly = threadIdx.x >> 4;
+    const int f = blockIdx.z;
+    const int n = a.first_frame + f;
+    const int r = a.radius;
+
+    const int bx0 = blockIdx.x * 16;
+    const int by0 = blockIdx.y * tby;
+    const int bxi = bx0 + lx;
+    const int byi = by0 + ly;
+    const bool active = bxi < a.nbx && byi < a.nby;
+    const int x = min(min(bxi, a.nbx - 1) * a.bs, a.W - 8);
+    const int y = min(min(byi, a.nby - 1) * a.bs, a.H - 8);
+
+    TileGeom g;
+    g.tx0 = min(bx0 * a.bs, a.W - 8) - a.margin;
+    g.ty0 = min(by0 * a.bs, a.H - 8) - a.margin;
+    g.tw = a.tile_w;
+    g.th = a.tile_h;
+    g.bs = a.bs;
+    g.wq = a.tile_wq;
+    g.hp = a.tile_hp;
+
+    auto slot_plane = [&](int z) -> const float* {
+        int fi = clampi(n - r + z, 0, a.num_frames - 1);
+        return a.clip + (size_t)fi * a.frame_stride;
+    };
+
+    stage_tile(lds, g, slot_plane(r), a.W, a.H, a.stride);
+    __syncthreads();
+
+    float rb[8][8];
+    {
+        const int tc = x - g.tx0;
+        const int rt = y - g.ty0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float* colp = lds + tile_col_base(g, tc + c) + rt;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) rb[j][c] = colp[j];
+        }
+    }
+
+    unsigned cnt = 0;
+    Best8 best;
+    best.init(kFltMax, pack_pos(x, y, r));
+    scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+
+    if (r > 0) {
+        uint32_t centre[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) centre[i] = best.p[i];
+        for (int dir = -1; dir <= 1; dir += 2) {
+            uint32_t last[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) last[i] = centre[i];
+            for (int t = 1; t <= r; ++t) {
+                const int z = r + dir * t;
+                __syncthreads();
+                stage_tile(lds, g, slot_plane(z), a.W, a.H, a.stride);
+                __syncthreads();
+                Best8 fr;
+                fr.init(kFltMax, pack_pos(0, 0, z));
+                for (int i = 0; i < a.ps_num; ++i) {
+                    uint32_t p = last[0];
+#pragma unroll
+                    for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                    scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
+                }
+                for (int i = 0; i < a.ps_num; ++i) {
+                    const float e = fr.err_at(i);
+                    if (e < best.e[7]) best.insert(e, fr.pos_at(i));
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) last[i] = fr.p[i];
+            }
+        }
+    }
+
+    // centre insert
+    const uint32_t cpos = pack_pos(x, y, r);
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) found |= best.p[i] == cpos;
+    if (!found) {
+#pragma unroll
+        for (int i = 7; i >= 1; --i) best.p[i] = best.p[i - 1];
+        best.p[0] = cpos;
+    }
+
+    if constexpr (kCount) {
+        if (active) atomicAdd(a.ncand, (unsigned long long)cnt);
+    }
+    if (active && a.groups) {
+        uint4* out = reinterpret_cast<uint4*>(
+            a.groups + ((size_t)f * a.nblocks + (size_t)byi * a.nbx + bxi) * 8);
+        out[0] = make_uint4(best.p[0], best.p[1], best.p[2], best.p[3]);
+        out[1] = make_uint4(best.p[4], best.p[5], best.p[6], best.p[7]);
+    }
+}
+
+// ---------------------------------------------------------------- filtering
+
+// Transpose the 8x8 blocks of each group between "lane k holds column k"
+// (v[i][j]) and "lane t holds row t" (v[i][kk]) through LDS.
+__device__ __forceinline__ void transpose8(float (&v)[8][8], float* __restrict__ buf, int g, int k) {
+    float* gb = buf + g * kTransposeGroupStride;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) gb[i * 64 + j * 8 + k] = v[i][j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) v[i][kk] = gb[i * 64 + k * 8 + kk];
+}
+
+// Forward 3-D transform: along j (in lane, column layout), transpose, along
+// k and along i (in lane, row layout). Leaves the coefficients in row layout.
+__device__ __forceinline__ void forward3d(float (&v)[8][8], float* buf, int g, int k) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dct8<false>(v[i]);
+    transpose8(v, buf, g, k);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dct8<false>(v[i]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dct8_strided<false>(v, c);
+}
+
+// Inverse 3-D transform from row layout: along i, along k, transpose, along j.
+__device__ __forceinline__ void inverse3d(float (&v)[8][8], float* buf, int g, int k) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dct8_strided<true>(v, c);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dct8<true>(v[i]);
+    transpose8(v, buf, g, k);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dct8<true>(v[i]);
+}
+
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(64) void bm3d_filter(FilterArgs a) {
+    __shared__ float buf[8 * kTransposeGroupStride];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3;
+    const int k = lane & 7;
+    const int f = blockIdx.y;
+    const int n = a.first_frame + f;
+    const int tw = 2 * a.radius + 1;
+    int blk = blockIdx.x * 8 + g;
+    const bool active = blk < a.nblocks;
+    blk = min(blk, a.nblocks - 1);
+
+    uint32_t pk[8];
+    {
+        const uint4* gp = reinterpret_cast<const uint4*>(a.groups + ((size_t)f * a.nblocks + blk) * 8);
+        uint4 q0 = gp[0], q1 = gp[1];
+        pk[0] = q0.x; pk[1] = q0.y; pk[2] = q0.z; pk[3] = q0.w;
+        pk[4] = q1.x; pk[5] = q1.y; pk[6] = q1.z; pk[7] = q1.w;
+    }
+    size_t off[8];  // element offset of member i (column k) within a plane-slot stack
+    int zz[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        // positions come from bm3d_match and are in range; the clamps only
+        // keep a corrupted list from addressing outside the planes
+        const int z = min(pos_z(pk[i]), 2 * a.radius);
+        const int px = min(pos_x(pk[i]), a.W - 8);
+        const int py = min(pos_y(pk[i]), a.H - 8);
+        pk[i] = pack_pos(px, py, z);
+        zz[i] = z;
+        const int fi = clampi(n - a.radius + z, 0, a.num_frames - 1);
+        off[i] = (size_t)fi * a.frame_stride + (size_t)py * a.stride + px + k;
+    }
+
+    const float E = a.extractor;
+    const int nplanes = kChroma ? 3 : 1;
+    for (int p = 0; p < nplanes; ++p) {
+        const float sigma = a.sigma[p];
+        if (kChroma && sigma < kFltEpsilon) continue;  // source/kernel.cu:657-668
+        const float* src = a.src + (size_t)p * a.plane_stride;
+        float v[8][8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[i][j] = src[off[i] + (size_t)j * a.stride];
+        forward3d(v, buf, g, k);
+
+        float w;
+        if constexpr (kFinal) {
+            const float* ref = a.ref + (size_t)p * a.plane_stride;
+            float b[8][8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) b[i][j] = ref[off[i] + (size_t)j * a.stride];
+            forward3d(b, buf, g, k);
+            // Wiener shrinkage (source/kernel.cu:284-326); lane k now holds row k.
+            const float s2 = sigma * sigma;
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float sq = b[i][c] * b[i][c];
+                    float coef = sq / (sq + s2);
+                    if (i == 0 && c == 0) coef = (k == 0) ? 1.0f : coef;
+                    acc = fma_(coef, coef, acc);
+                    v[i][c] = (v[i][c] * coef) * (1.0f / 4096.0f);
+                }
+            acc = acc + __shfl_xor(acc, 1, 8);
+            acc = acc + __shfl_xor(acc, 2, 8);
+            acc = acc + __shfl_xor(acc, 4, 8);
+            w = 1.0f / acc;
+        } else {
+            // hard thresholding (source/kernel.cu:208-252)
+            int kept = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float val = v[i][c];
+                    const float thr = (i == 0 && c == 0 && k == 0) ? 0.0f : sigma;
+                    const bool keep = fabsf(val) >= thr;
+                    kept += keep ? 1 : 0;
+                    v[i][c] = keep ? val * (1.0f / 4096.0f) : 0.0f;
+                }
+            kept += __shfl_xor(kept, 1, 8);
+            kept += __shfl_xor(kept, 2, 8);
+            kept += __shfl_xor(kept, 4, 8);
+            w = 1.0f / (float)kept;
+        }
+        inverse3d(v, buf, g, k);
+
+        if (active) {
+            // aggregation with pre-rounding (source/kernel.cu:717-756)
+            float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
+            float* corner = a.corner + (size_t)f * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
+            const float wr = (w + E) - E;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const size_t o = (size_t)pos_y(pk[i]) * a.stride + pos_x(pk[i]) + k;
+                float* num = res + (size_t)(2 * zz[i]) * a.plane_size + o;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    float val = w * v[i][j];
+                    val = (val + E) - E;
+                    unsafeAtomicAdd(num + (size_t)j * a.stride, val);
+                }
+                if (k == 0) unsafeAtomicAdd(corner + (size_t)zz[i] * a.plane_size + o, wr);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- elementwise
+
+// Zeroes grid.y runs of n floats, `step` floats apart (a fill kernel rather
+// than hipMemsetAsync, so the sequence can be captured in a graph).
+__global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
+    float* q = p + (size_t)blockIdx.y * step;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t inc = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += inc) q[i] = 0.0f;
+}
+
+// den[y][x] = sum over a, b in [0, 8) of corner[y - a][x - b]: each member
+// block's weight added once at its corner covers its 8x8 footprint. Rows of
+// a 64 x 8 output tile; the sum runs rows-then-columns in a fixed order.
+// Grid: (ceil(W/64), ceil(H/8), frames * planes * slots).
+__global__ __launch_bounds__(256) void bm3d_den_box(DenArgs a) {
+    __shared__ float rowsum[15][64];
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;  // 0..3
+    const int x = blockIdx.x * 64 + tx;
+    const int y0 = blockIdx.y * 8;
+    const int fps = blockIdx.z;
+    const int slot = fps % a.slots;
+    const int p = (fps / a.slots) % a.nplanes;
+    const int f = fps / (a.slots * a.nplanes);
+    const float* cmap = a.corner + (size_t)f * a.corner_frame_stride +
+                        (size_t)(p * a.slots + slot) * a.plane_size;
+    float* den = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride +
+                 (size_t)(slot * 2 + 1) * a.plane_size;
+    // horizontal 8-sums for rows y0-7 .. y0+7
+    for (int rr = ty; rr < 15; rr += 4) {
+        const int yy = y0 - 7 + rr;
+        float s = 0.0f;
+        if (yy >= 0 && yy < a.H) {
+            const float* row = cmap + (size_t)yy * a.stride;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int xx = x - b;
+                s = s + ((xx >= 0 && xx < a.W) ? row[xx] : 0.0f);
+            }
+        }
+        rowsum[rr][tx] = s;
+    }
+    __syncthreads();
+    for (int jj = ty; jj < 8; jj += 4) {
+        const int y = y0 + jj;
+        if (y < a.H && x < a.W) {
+            float s = 0.0f;
+#pragma unroll
+            for (int aa = 0; aa < 8; ++aa) s = s + rowsum[jj + 7 - aa][tx];
+            den[(size_t)y * a.stride + x] = s;
+        }
+    }
+}
+
+// out = num / den per pixel (r = 0; source/source.cpp:170-188).
+__global__ void bm3d_divide(DivideArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int fp = blockIdx.z;  // frame * planes + plane
+    if (x >= a.W) return;
+    const int f = fp / a.nplanes, p = fp % a.nplanes;
+    const float* num = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride +
+                       (size_t)y * a.stride + x;
+    const float nv = num[0];
+    const float dv = num[a.plane_size];
+    a.dst[(size_t)f * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] = nv / dv;
+}
+
+// VAggregate (source/source.cpp:816-846): output frame n sums slot
+// s(i) = clamp(2r - i, n - N + 1 + r, n + r) of the stack of frame
+// clamp(n - r + i, 0, N - 1), i ascending from 0, then divides.
+__global__ void bm3d_vaggregate(VAggArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int fp = blockIdx.z;  // out-frame-in-batch * planes + plane
+    if (x >= a.W) return;
+    const int fo = fp / a.nplanes, p = fp % a.nplanes;
+    const int n = a.first_frame + fo;
+    const int r = a.radius;
+    const int tw = 2 * r + 1;
+    float num = 0.0f, den = 0.0f;
+    for (int i = 0; i < tw; ++i) {
+        const int fi = clampi(n - r + i, 0, a.num_frames - 1);
+        const int s = clampi(2 * r - i, n - a.num_frames + 1 + r, n + r);
+        const float* st = a.stacks + (size_t)(fi - a.stack_first) * a.stack_frame_stride +
+                          (size_t)p * a.stack_plane_stride + (size_t)(2 * s) * a.plane_size +
+                          (size_t)y * a.stride + x;
+        num = num + st[0];
+        den = den + st[a.plane_size];
+    }
+    a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] = num / den;
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
+    if (a.ncand) hipLaunchKernelGGL(bm3d_match<true>, grid, dim3(16 * tby), lds_bytes, s, a);
+    else hipLaunchKernelGGL(bm3d_match<false>, grid, dim3(16 * tby), lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+hipError_t set_match_lds_limit(size_t bytes) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bm3d_match<false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&bm3d_match<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, hipStream_t s) {
+    dim3 grid((a.nblocks + 7) / 8, frames);
+    if (final_) {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<true, false>), grid, dim3(64), 0, s, a);
+    } else {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<false, true>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<false, false>), grid, dim3(64), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s) {
+    if (n == 0 || count == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(bm3d_zero, dim3((unsigned)blocks, count), dim3(256), 0, s, p, n, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s) {
+    dim3 grid((a.W + 63) / 64, (a.H + 7) / 8, count);
+    hipLaunchKernelGGL(bm3d_den_box, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
+    dim3 grid((a.W + 255) / 256, a.H, count);
+    hipLaunchKernelGGL(bm3d_divide, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s) {
+    dim3 grid((a.W + 255) / 256, a.H, count);
+    hipLaunchKernelGGL(bm3d_vaggregate, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace bm3d
