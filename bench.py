@@ -8,19 +8,26 @@ block_step=4, bm_range=9, ps_num=2, ps_range=4, radius=1:
     basic = BM3Dv2(src, radius=1)            -> BM3D stacks + VAggregate
     final = BM3Dv2(src, ref=basic, radius=1) -> BM3D stacks + VAggregate
 The clip is resident in HBM before the timed region; value = frames/s of
-the whole two-pass job over K steps.
+the whole two-pass job over K steps, for the mixed-precision path (FP16
+block-matching preselection with exact FP32 rescoring). The plain FP32 path
+is timed the same way beside it (fp32_value, speedup_vs_fp32).
 
-roofline: the block-matching kernel (dominant). Algorithmic FLOPs per launch
-= 192 x (in-plane candidates evaluated in that launch), the candidate count
-taken from the kernel's own counter; duration from HIP events bracketing
-each launch on its stream inside the timed region.
+roofline: the block-matching kernel (dominant) of the reported path.
+Algorithmic FLOPs per launch = 192 x (in-plane candidates evaluated in that
+launch), the candidate count taken from the kernel's own counter; duration
+from HIP events bracketing each launch on its stream inside the timed region.
+
+quality: PSNR of both paths against the clean clip; share of group lists
+the mixed path shares with the FP32 path; and, on a 192x160 crop of the
+first 5 frames, max-abs error and PSNR delta of both GPU paths against the
+FP64 CPU oracle running the same pipeline.
 
 cpu_baseline: the CPU oracle (a C restatement of the reference algorithm),
-frame-parallel over strips on host threads, on a bounded sample: one
-basic + one final BM3D window (radius 1) for 2 frames, each split into 8
-full-width strips of 135 rows.
+frame-parallel over strips on host threads, on a bounded sample: one basic +
+one final BM3D window (radius 1) for 2 frames, each split into 8 full-width
+strips of 135 rows.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|mixed]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp32]
 """
 from __future__ import annotations
 
@@ -38,6 +45,10 @@ sys.path.insert(0, os.path.join(ROOT, "vapoursynth-bm3dcuda_amd"))
 H, W, N = 1080, 1920, 30
 PARAMS = dict(sigma=15.0, block_step=4, bm_range=9, radius=1, ps_num=2, ps_range=4)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(noisy, basic, threads):
@@ -69,10 +80,95 @@ def cpu_baseline(noisy, basic, threads):
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         list(ex.map(work, jobs))
     dt = time.perf_counter() - t0
-    return {"value": len(frames) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+    return {"value": round(len(frames) / dt, 4), "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C, -O2) two-pass windows (basic+final, r=1) of {len(frames)} "
                       f"1080p frames as {len(jobs)} strips of {rows} rows on {threads} threads; "
                       f"{dt:.2f} s wall"}
+
+
+class Pipeline:
+    """Two-pass BM3Dv2 (radius 1) over a device-resident clip."""
+
+    def __init__(self, core, src, precision):
+        self.core = core
+        self.src = src
+        n, _, h, w = src.shape
+        self.n = n
+        p = PARAMS
+        self.pb = core.Plan(w, h, p["sigma"], p["block_step"], p["bm_range"], p["radius"],
+                            p["ps_num"], p["ps_range"], final=False, precision=precision)
+        self.pf = core.Plan(w, h, p["sigma"], p["block_step"], p["bm_range"], p["radius"],
+                            p["ps_num"], p["ps_range"], final=True, precision=precision)
+        self.stacks = self.pb.new_stacks(n)
+        import torch
+        self.basic = torch.empty_like(src)
+        self.final = torch.empty_like(src)
+
+    def step(self, stream=None):
+        r = PARAMS["radius"]
+        self.pb.execute(self.src, out=self.stacks, stream=stream)
+        self.core.vaggregate(self.stacks, r, self.n, out=self.basic, stream=stream)
+        self.pf.execute(self.src, self.basic, out=self.stacks, stream=stream)
+        self.core.vaggregate(self.stacks, r, self.n, out=self.final, stream=stream)
+
+
+def time_path(core, capi, src, precision, steps, warmup):
+    import torch
+    pipe = Pipeline(core, src, precision)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        pipe.step(stream)
+    torch.cuda.synchronize()
+    capi.profile_reset()
+    capi.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.step(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    capi.profile_enable(False)
+    match_ms, match_n = capi.profile_read(capi.KERNEL_MATCH)
+    filt_ms, filt_n = capi.profile_read(capi.KERNEL_FILTER)
+    capi.profile_reset()
+    ncand = pipe.pb.count_candidates(src) + pipe.pf.count_candidates(src, pipe.basic)
+    flops_step = 192.0 * ncand
+    avg_ms = match_ms / match_n
+    flops_launch = flops_step / (match_n / steps)
+    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    log(f"{precision}: {steps} steps in {dt:.3f}s, match {match_ms / steps:.2f} ms/step, "
+        f"filter {filt_ms / steps:.2f} ms/step")
+    return {
+        "fps": src.shape[0] * steps / dt, "ms_per_step": dt / steps * 1e3,
+        "match_ms_step": match_ms / steps, "filter_ms_step": filt_ms / steps,
+        "avg_launch_ms": avg_ms, "flops_launch": flops_launch, "achieved_tflops": achieved,
+        "ncand": ncand, "pipe": pipe,
+    }
+
+
+def oracle_crop_errors(core, noisy, clean):
+    """Both GPU paths vs the FP64 oracle on a 192x160 crop of 5 frames."""
+    import numpy as np
+    import torch
+    from bm3d_amd import synth
+    from oracle import oracle as O
+
+    crop_n = np.ascontiguousarray(noisy[:5, :, 200:360, 400:592])
+    crop_c = np.ascontiguousarray(clean[:5, :, 200:360, 400:592])
+    p = PARAMS
+    op = O.Params(sigma=(p["sigma"],), block_step=p["block_step"], bm_range=p["bm_range"],
+                  radius=p["radius"], ps_num=p["ps_num"], ps_range=p["ps_range"])
+    _, f64 = O.two_pass(crop_n.astype(np.float64), op, dtype=np.float64)
+    ref_psnr = synth.psnr(f64, crop_c)
+    out = {"oracle_fp64_psnr_db": round(ref_psnr, 4)}
+    src = torch.from_numpy(crop_n).cuda()
+    for prec in ("fp32", "mixed"):
+        pipe = Pipeline(core, src, prec)
+        pipe.step()
+        fin = pipe.final.cpu().numpy()
+        out[prec] = {"max_abs_vs_fp64": float(np.abs(fin - f64).max()),
+                     "delta_psnr_db": round(synth.psnr(fin, crop_c) - ref_psnr, 5)}
+    return out
 
 
 def main():
@@ -80,9 +176,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed"],
+                    help="path reported as value (the other is timed beside it)")
     ap.add_argument("--frames", type=int, default=N)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="time only --precision")
     ap.add_argument("--cpu-threads", type=int, default=8)
     args = ap.parse_args()
 
@@ -103,71 +201,36 @@ def main():
     nf = args.frames
     t0 = time.perf_counter()
     clean, noisy = synth.make_clip(H, W, nf, 1, noise_sd=15.0, seed=1, motion=True)
-    gen_s = time.perf_counter() - t0
-    print(f"[bench] generated {nf} frames in {gen_s:.1f}s sha256={synth.sha256(noisy)[:16]}",
-          file=sys.stderr, flush=True)
-
+    log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s sha256={synth.sha256(noisy)[:16]}")
     src = torch.from_numpy(noisy).cuda()
-    prm = dict(PARAMS)
-    plan_b = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
-                       prm["ps_num"], prm["ps_range"], final=False, precision=args.precision)
-    plan_f = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
-                       prm["ps_num"], prm["ps_range"], final=True, precision=args.precision)
-    stacks = plan_b.new_stacks(nf)
-    basic = torch.empty_like(src)
-    final = torch.empty_like(src)
-    stream = torch.cuda.current_stream()
 
-    def step():
-        plan_b.execute(src, out=stacks, stream=stream)
-        core.vaggregate(stacks, prm["radius"], nf, out=basic, stream=stream)
-        plan_f.execute(src, basic, out=stacks, stream=stream)
-        core.vaggregate(stacks, prm["radius"], nf, out=final, stream=stream)
+    paths = [args.precision] if args.no_compare else ["fp32", "mixed"]
+    res = {p: time_path(core, _capi, src, p, args.steps, args.warmup) for p in paths}
+    main_r = res[args.precision]
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    quality = {}
+    for p, r in res.items():
+        quality[f"psnr_final_{p}_db"] = round(synth.psnr(r["pipe"].final.cpu().numpy(), clean), 4)
+    quality["psnr_noisy_db"] = round(synth.psnr(noisy, clean), 4)
+    if "fp32" in res and "mixed" in res:
+        g32 = res["fp32"]["pipe"].pb.match(src, first=0, count=4).cpu().numpy()
+        gmx = res["mixed"]["pipe"].pb.match(src, first=0, count=4).cpu().numpy()
+        quality["mixed_group_agreement_basic"] = float(np.all(g32 == gmx, axis=-1).mean())
+        b32 = res["fp32"]["pipe"].basic
+        g32f = res["fp32"]["pipe"].pf.match(src, b32, first=0, count=4).cpu().numpy()
+        gmxf = res["mixed"]["pipe"].pf.match(src, b32, first=0, count=4).cpu().numpy()
+        quality["mixed_group_agreement_final"] = float(np.all(g32f == gmxf, axis=-1).mean())
+        log("oracle FP64 crop comparison ...")
+        quality["vs_fp64_oracle_crop"] = oracle_crop_errors(core, noisy, clean)
 
-    _capi.profile_reset()
-    _capi.profile_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    _capi.profile_enable(False)
-    match_ms, match_n = _capi.profile_read(_capi.KERNEL_MATCH)
-    filt_ms, filt_n = _capi.profile_read(_capi.KERNEL_FILTER)
-    _capi.profile_reset()
-    print(f"[bench] timed {args.steps} steps in {dt:.3f}s", file=sys.stderr, flush=True)
-
-    # algorithmic work of the matching kernel (outside the timed region)
-    ncand_b = plan_b.count_candidates(src)
-    ncand_f = plan_f.count_candidates(src, basic)
-    flops_match_step = 192.0 * (ncand_b + ncand_f)
-    launches_per_step = match_n / args.steps
-    avg_ms = match_ms / match_n
-    flops_per_launch = flops_match_step / launches_per_step
-    achieved_tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
-
-    fin = final.cpu().numpy()
-    bas = basic.cpu().numpy()
-    quality = {
-        "psnr_noisy_db": round(synth.psnr(noisy, clean), 3),
-        "psnr_basic_db": round(synth.psnr(bas, clean), 3),
-        "psnr_final_db": round(synth.psnr(fin, clean), 3),
-    }
-    fps = nf * args.steps / dt
     out = {
         "metric": "1080p frames/s, two-pass BM3Dv2 radius=1 (V-BM3D)",
-        "value": round(fps, 3),
+        "value": round(main_r["fps"], 3),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(main_r["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -176,26 +239,31 @@ def main():
         "config": {"workload": "GRAYS 1920x1080 x30 frames, two-pass BM3Dv2 radius=1, sigma=15, "
                                "block_step=4, bm_range=9, ps_num=2, ps_range=4",
                    "frames": nf, "precision": args.precision},
-        "mpixel_per_s": round(fps * W * H / 1e6, 2),
+        "mpixel_per_s": round(main_r["fps"] * W * H / 1e6, 2),
         "roofline": {
             "bound": "mfma",
-            "compute_pipe": "fp32 VALU (packed FP32 disabled; attainable 78.6 TFLOP/s)",
+            "compute_pipe": "VALU (FP32 exact / packed FP16 preselection); packed FP32 disabled",
             "kernel": "bm3d_match",
-            "achieved": round(achieved_tflops, 3),
+            "achieved": round(main_r["achieved_tflops"], 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
+            "frac": round(main_r["achieved_tflops"] / PEAK_FP32_TFLOPS, 4),
             "traffic": None,
-            "flops_per_launch": flops_per_launch,
-            "avg_launch_ms": round(avg_ms, 4),
-            "candidates_per_step": ncand_b + ncand_f,
+            "flops_per_launch": main_r["flops_launch"],
+            "avg_launch_ms": round(main_r["avg_launch_ms"], 4),
+            "candidates_per_step": main_r["ncand"],
         },
-        "kernel_ms_per_step": {"bm3d_match": round(match_ms / args.steps, 3),
-                               "bm3d_filter": round(filt_ms / args.steps, 3)},
+        "kernel_ms_per_step": {p: {"bm3d_match": round(r["match_ms_step"], 3),
+                                   "bm3d_filter": round(r["filter_ms_step"], 3)}
+                               for p, r in res.items()},
         "quality": quality,
     }
+    if "fp32" in res and "mixed" in res:
+        out["fp32_value"] = round(res["fp32"]["fps"], 3)
+        out["speedup_vs_fp32"] = round(res["mixed"]["fps"] / res["fp32"]["fps"], 4)
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        log("cpu baseline ...")
+        bas = res[args.precision]["pipe"].basic.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(noisy, bas, args.cpu_threads)
     else:
         out["cpu_baseline"] = None

@@ -4,8 +4,9 @@ Bars:
 * group lists (block matching, temporal search, centre insert): identical;
 * stacks with extractor_exp set: bit-identical (the pre-rounded atomic sums
   are exact, so their order does not matter);
-* stacks without extractor_exp: float atomics accumulate in hardware order,
-  so num/den agree to a relative 2e-6 and the denoised output to 1e-5 abs.
+* stacks without extractor_exp: weighted sums accumulate in 2^-22 fixed
+  point and weights in float atomics of unspecified order, so stacks agree
+  to rtol 2e-6 + atol 5e-6 (32 contributions x 2^-23 rounding each).
 """
 import numpy as np
 import pytest
@@ -108,7 +109,7 @@ def test_two_pass_stacks(case, extractor_exp):
     if extractor_exp:
         np.testing.assert_array_equal(got_b, want_b)
     else:
-        np.testing.assert_allclose(got_b, want_b, rtol=2e-6, atol=1e-6)
+        np.testing.assert_allclose(got_b, want_b, rtol=2e-6, atol=5e-6)
     # final pass on the oracle's basic estimate (same ref for both sides)
     basic = np.empty_like(noisy)
     for n in range(N):
@@ -127,7 +128,7 @@ def test_two_pass_stacks(case, extractor_exp):
     if extractor_exp:
         np.testing.assert_array_equal(got_f, want_f)
     else:
-        np.testing.assert_allclose(got_f, want_f, rtol=2e-6, atol=1e-6)
+        np.testing.assert_allclose(got_f, want_f, rtol=2e-6, atol=5e-6)
 
 
 def test_vaggregate_and_aggregate_match_oracle():
@@ -157,3 +158,44 @@ def test_deterministic_with_extractor():
     a = plan.execute(src).cpu().numpy()
     b = plan.execute(src).cpu().numpy()
     assert synth.sha256(a) == synth.sha256(b)
+
+
+# ---------------------------------------------------------- mixed precision
+MIXED_CASES = [c for c in CASES if c[5]["bm_range"] <= 63]
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=[i for i, c in zip(_ids(), CASES) if c in MIXED_CASES])
+def test_mixed_groups_equal_fp32(case):
+    """FP16 preselection + exact FP32 rescoring reproduces the FP32 path's
+    group lists (the preselection keeps 16 candidates for 8 slots)."""
+    core = _gpu()
+    H, W, N, P, chroma, prm = case
+    _, noisy = synth.make_clip(H, W, N, P, noise_sd=prm["sigma"][0], seed=3, motion=True, yuv=chroma)
+    src = torch.from_numpy(noisy).cuda()
+    p32 = _plan(core, W, H, prm, False, chroma)
+    pmx = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
+                    prm["ps_num"], prm["ps_range"], chroma=chroma, precision="mixed")
+    g32 = p32.match(src).cpu().numpy()
+    gmx = pmx.match(src).cpu().numpy()
+    same = np.all(g32 == gmx, axis=-1).mean()
+    assert same >= 0.999, f"only {same:.4%} of group lists agree"
+
+
+def test_mixed_two_pass_close_to_fp64():
+    core = _gpu()
+    H, W, N = 96, 128, 4
+    clean, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=21, motion=True)
+    prm = dict(sigma=15.0, block_step=4, bm_range=6, radius=1, ps_num=2, ps_range=3)
+    src = torch.from_numpy(noisy).cuda()
+    out = {}
+    for prec in ("fp32", "mixed"):
+        pb = core.Plan(W, H, precision=prec, **prm)
+        pf = core.Plan(W, H, final=True, precision=prec, **prm)
+        basic = core.vaggregate(pb.execute(src), 1, N)
+        out[prec] = core.vaggregate(pf.execute(src, basic), 1, N).cpu().numpy()
+    p = O.Params(sigma=(15.0,), block_step=4, bm_range=6, radius=1, ps_num=2, ps_range=3)
+    _, f64 = O.two_pass(noisy.astype(np.float64), p, dtype=np.float64)
+    ref_psnr = synth.psnr(f64, clean)
+    for prec, o in out.items():
+        assert abs(synth.psnr(o, clean) - ref_psnr) <= 0.05, prec
+        assert np.abs(o - f64).max() < 0.05, prec

@@ -87,8 +87,14 @@ struct bm3d_plan {
     size_t lds_bytes = 0;
     long long plane_size = 0, stack_size = 0, corner_frame = 0;
     int device = 0;
+    // filter work-group tiling and LDS accumulation region
+    int ftbx = 16, ftby = 8, fntx = 0, fnty = 0, frm = 0, frw = 0, frh = 0;
+    size_t filter_lds_bytes = 0;
+    long long scratch_frame = 0;  // floats of region scratch per frame
     uint32_t* groups = nullptr;
     float* corner = nullptr;
+    int* scratch = nullptr;
+    float fx_unit = 0.0f;  // weighted-sum fixed-point unit
     int cap_frames = 0;
 };
 
@@ -111,15 +117,48 @@ void tile_geometry(bm3d_plan* p, int tby) {
     p->lds_bytes = (size_t)bs * p->tile_wq * p->tile_hp * sizeof(float);
 }
 
+// LDS bytes of the filter kernel for a tile of tbx x tby blocks and margin rm.
+size_t filter_lds(const bm3d_plan* p, int tbx, int tby, int rm, int* rw, int* rh) {
+    const int bs = p->d.block_step;
+    *rw = (tbx - 1) * bs + 8 + 2 * rm;
+    *rh = (tby - 1) * bs + 8 + 2 * rm;
+    return (size_t)p->tw * *rw * *rh * sizeof(float);
+}
+
+// Filter tiling: the largest accumulation margin up to the full search reach
+// (bm_range + radius * ps_range) whose region fits the LDS budget; members
+// beyond a reduced margin fall back to global atomics.
+void filter_geometry(bm3d_plan* p, size_t budget) {
+    const int full = p->margin;
+    const int tiles[][2] = {{8, 8}, {8, 4}};
+    for (auto& t : tiles) {
+        for (int rm = full; rm >= 0; --rm) {
+            int rw, rh;
+            size_t b = filter_lds(p, t[0], t[1], rm, &rw, &rh);
+            if (b <= budget && (rm == full || (t[0] == 8 && t[1] == 4))) {
+                p->ftbx = t[0]; p->ftby = t[1]; p->frm = rm; p->frw = rw; p->frh = rh;
+                p->filter_lds_bytes = b;
+                p->fntx = (p->nbx + t[0] - 1) / t[0];
+                p->fnty = (p->nby + t[1] - 1) / t[1];
+                p->scratch_frame = (long long)p->fntx * p->fnty * p->nplanes * p->tw * rw * rh;
+                return;
+            }
+        }
+    }
+}
+
 int ensure_scratch(bm3d_plan* p, int frames) {
     if (frames <= p->cap_frames) return BM3D_OK;
     if (p->groups) (void)hipFree(p->groups);
     if (p->corner) (void)hipFree(p->corner);
+    if (p->scratch) (void)hipFree(p->scratch);
     p->groups = nullptr;
     p->corner = nullptr;
+    p->scratch = nullptr;
     p->cap_frames = 0;
     BM3D_HIP(hipMalloc(&p->groups, (size_t)frames * p->nblocks * 8 * sizeof(uint32_t)));
     BM3D_HIP(hipMalloc(&p->corner, (size_t)frames * p->corner_frame * sizeof(float)));
+    BM3D_HIP(hipMalloc(&p->scratch, (size_t)frames * p->scratch_frame * sizeof(int)));
     p->cap_frames = frames;
     return BM3D_OK;
 }
@@ -156,8 +195,9 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         return fail(BM3D_EINVAL, "BM3D: \"extractor_exp\" out of range");
     if (d.precision != BM3D_PRECISION_FP32 && d.precision != BM3D_PRECISION_MIXED)
         return fail(BM3D_EINVAL, "BM3D: unknown precision");
-    if (d.precision == BM3D_PRECISION_MIXED)
-        return fail(BM3D_EUNSUPPORTED, "BM3D: mixed precision is not available in this build");
+    if (d.precision == BM3D_PRECISION_MIXED &&
+        (d.bm_range > bm3d::kMixedMaxRange || d.ps_range > bm3d::kMixedMaxPsRange))
+        return fail(BM3D_EUNSUPPORTED, "BM3D: mixed precision needs bm_range <= 63 and ps_range <= 31");
 
     auto* p = new bm3d_plan;
     p->d = d;
@@ -168,6 +208,10 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     for (int i = 0; i < 3; ++i) p->sigma[i] = d.sigma[i] * f;
     // extractor = 2^e, source/source.cpp:623-629
     p->extractor = d.extractor_exp ? std::ldexp(1.0f, d.extractor_exp) : 0.0f;
+    // Fixed-point unit of the LDS accumulation: with extractor 2^e, every
+    // pre-rounded value in (-E/2, 2E) is a multiple of 2^(e-24), so sums are
+    // exact; otherwise 2^-22 (1.2e-7 rounding, sums up to +-512).
+    p->fx_unit = d.extractor_exp ? std::ldexp(1.0f, d.extractor_exp - 24) : std::ldexp(1.0f, -22);
     p->nplanes = p->d.chroma ? 3 : 1;
     p->tw = 2 * d.radius + 1;
     p->nbx = count_positions(d.width, d.block_step);
@@ -202,6 +246,18 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
             return hip_fail(e, "hipFuncSetAttribute");
         }
     }
+    filter_geometry(p, std::min<size_t>((size_t)max_lds, 52 * 1024));
+    if (p->filter_lds_bytes == 0 || p->filter_lds_bytes > (size_t)max_lds) {
+        delete p;
+        return fail(BM3D_EUNSUPPORTED, "BM3D: no filter tiling fits the device's LDS");
+    }
+    if (p->filter_lds_bytes > 65536) {
+        e = bm3d::set_filter_lds_limit(p->filter_lds_bytes);
+        if (e != hipSuccess) {
+            delete p;
+            return hip_fail(e, "hipFuncSetAttribute");
+        }
+    }
     *out = p;
     return BM3D_OK;
 }
@@ -210,6 +266,7 @@ void bm3d_plan_destroy(bm3d_plan* p) {
     if (!p) return;
     if (p->groups) (void)hipFree(p->groups);
     if (p->corner) (void)hipFree(p->corner);
+    if (p->scratch) (void)hipFree(p->scratch);
     delete p;
 }
 
@@ -244,12 +301,18 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
         const int nf = std::min(batch, count - f0);
         const int n0 = first_frame + f0;
         float* res = d_res + (size_t)f0 * res_frame_stride;
-        if (res_plane_stride == plane_stack) {
-            BM3D_HIP(bm3d::launch_zero(res, (size_t)p->nplanes * plane_stack, (size_t)res_frame_stride, nf, stream));
-        } else {
-            for (int f = 0; f < nf; ++f)
-                BM3D_HIP(bm3d::launch_zero(res + (size_t)f * res_frame_stride, (size_t)plane_stack,
-                                           (size_t)res_plane_stride, p->nplanes, stream));
+        // Stacks are fully written by finalize (sums) and den-box (weights);
+        // they start from zero only when members can land outside the LDS
+        // regions and go to global atomics.
+        const bool outliers = p->frm < p->margin;
+        if (outliers) {
+            if (res_plane_stride == plane_stack) {
+                BM3D_HIP(bm3d::launch_zero(res, (size_t)p->nplanes * plane_stack, (size_t)res_frame_stride, nf, stream));
+            } else {
+                for (int f = 0; f < nf; ++f)
+                    BM3D_HIP(bm3d::launch_zero(res + (size_t)f * res_frame_stride, (size_t)plane_stack,
+                                               (size_t)res_plane_stride, p->nplanes, stream));
+            }
         }
         BM3D_HIP(bm3d::launch_zero(p->corner, (size_t)nf * p->corner_frame, 0, 1, stream));
 
@@ -265,6 +328,7 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
         m.margin = p->margin;
         m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
         m.groups = p->groups;
+        m.mixed = d.precision == BM3D_PRECISION_MIXED;
         {
             ProfScope ps(0, stream);
             BM3D_HIP(bm3d::launch_match(m, p->tby, p->lds_bytes, nf, stream));
@@ -289,10 +353,32 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
         fa.res_plane_stride = res_plane_stride;
         fa.corner = p->corner;
         fa.corner_frame_stride = p->corner_frame;
+        fa.bs = d.block_step; fa.nbx = p->nbx; fa.nby = p->nby;
+        fa.tbx = p->ftbx; fa.tby = p->ftby; fa.ntx = p->fntx; fa.nty = p->fnty;
+        fa.rm = p->frm; fa.rw = p->frw; fa.rh = p->frh;
+        fa.scratch = p->scratch;
+        fa.fx_scale = 1.0f / p->fx_unit;
         {
             ProfScope ps(1, stream);
-            BM3D_HIP(bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, stream));
+            BM3D_HIP(bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream));
         }
+
+        bm3d::FinalizeArgs fz{};
+        fz.scratch = p->scratch;
+        fz.fx_unit = p->fx_unit;
+        fz.res = res;
+        fz.res_frame_stride = res_frame_stride;
+        fz.res_plane_stride = res_plane_stride;
+        fz.W = d.width; fz.H = d.height; fz.stride = d.stride;
+        fz.plane_size = p->plane_size;
+        fz.nplanes = p->nplanes;
+        fz.slots = p->tw;
+        fz.bs = d.block_step; fz.tbx = p->ftbx; fz.tby = p->ftby; fz.ntx = p->fntx; fz.nty = p->fnty;
+        fz.rm = p->frm; fz.rw = p->frw; fz.rh = p->frh;
+        fz.accumulate = outliers ? 1 : 0;
+        fz.chroma = d.chroma;
+        for (int i = 0; i < 3; ++i) fz.sigma[i] = p->sigma[i];
+        BM3D_HIP(bm3d::launch_finalize(fz, nf * p->nplanes * p->tw, stream));
 
         bm3d::DenArgs da{};
         da.corner = p->corner;
@@ -374,6 +460,7 @@ int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* r
     m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
     m.groups = nullptr;
     m.ncand = d_cnt;
+    m.mixed = d.precision == BM3D_PRECISION_MIXED;
     hipError_t e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st);
     if (e == hipSuccess && count > 0) e = bm3d::launch_match(m, p->tby, p->lds_bytes, count, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ncand, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
@@ -404,6 +491,7 @@ int bm3d_match(bm3d_plan* p, unsigned int* d_groups, const bm3d_clip* src, const
     m.margin = p->margin;
     m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
     m.groups = d_groups;
+    m.mixed = d.precision == BM3D_PRECISION_MIXED;
     BM3D_HIP(bm3d::launch_match(m, p->tby, p->lds_bytes, count, reinterpret_cast<hipStream_t>(stream)));
     return BM3D_OK;
 }

@@ -16,10 +16,12 @@ constexpr uint32_t kPosYMask = (1u << 13) - 1;
 constexpr int kMaxWidth = 1 << 14;
 constexpr int kMaxHeight = 1 << 13;
 constexpr int kMaxRadius = 15;
+// Limits of the mixed-precision matching path (scan indices fit 16 bits).
+constexpr int kMixedMaxRange = 63;
+constexpr int kMixedMaxPsRange = 31;
 
-// Floats per group in the filter kernel's transpose buffer (512 + padding
-// that spreads the 8 groups of a wave over different LDS banks).
-constexpr int kTransposeGroupStride = 520;
+// Waves per filter work-group (8 groups per wave).
+constexpr int kFilterWaves = 4;
 
 struct MatchArgs {
     const float* clip;       // plane used for matching, frame 0
@@ -33,6 +35,7 @@ struct MatchArgs {
     int tile_w, tile_h, tile_wq, tile_hp;
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
     unsigned long long* ncand;  // non-null: count in-plane candidates
+    int mixed;               // FP16 preselection + exact FP32 rescoring
 };
 
 struct FilterArgs {
@@ -51,6 +54,25 @@ struct FilterArgs {
     long long res_frame_stride, res_plane_stride;
     float* corner;           // [frames][planes][2r+1][H][stride]
     long long corner_frame_stride;
+    int bs, nbx, nby;
+    int tbx, tby, ntx, nty;  // work-group tile of reference blocks, tile grid
+    int rm, rw, rh;          // accumulation region: margin, width, height
+    int* scratch;            // [frames][nty][ntx][planes][2r+1][rh][rw], fixed point
+    float fx_scale;          // float -> fixed-point units (1 / fx_unit)
+};
+
+struct FinalizeArgs {
+    const int* scratch;
+    float fx_unit;           // value of one fixed-point unit
+    float* res;
+    long long res_frame_stride, res_plane_stride;
+    int W, H, stride;
+    long long plane_size;
+    int nplanes, slots;
+    int bs, tbx, tby, ntx, nty, rm, rw, rh;
+    int accumulate;          // add to the stack's existing values
+    int chroma;
+    float sigma[3];
 };
 
 struct DenArgs {
@@ -89,7 +111,10 @@ struct VAggArgs {
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_lds_limit(size_t bytes);
-hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, hipStream_t s);
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
+                         hipStream_t s);
+hipError_t set_filter_lds_limit(size_t bytes);
+hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s);
 // Zeroes `count` runs of `n` floats, `step` floats apart.
 hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s);
 hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s);

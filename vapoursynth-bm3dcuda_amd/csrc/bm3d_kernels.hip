@@ -171,7 +171,7 @@ __device__ __forceinline__ int tile_col_base(const TileGeom& g, int tc) {
 
 // Fill the tile from one plane; out-of-image pixels replicate the border
 // (they are only ever read for candidates that are masked off).
-__device__ void stage_tile(float* __restrict__ lds, const TileGeom& g,
+__device__ __forceinline__ void stage_tile(float* __restrict__ lds, const TileGeom& g,
                            const float* __restrict__ img, int W, int H, int stride) {
     const int n = g.tw * g.th;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
@@ -284,14 +284,220 @@ __device__ __forceinline__ void scan_window(const float* __restrict__ lds, const
     }
 }
 
+// ------------------------------------------------- mixed-precision matching
+//
+// Candidates are first scored approximately: the reference block and each
+// candidate column are offset by the reference block's mean and rounded to
+// FP16 (so flat regions keep their relative precision), differenced with
+// packed FP16 subtracts and squared-and-summed into FP32 with v_dot2_f32_f16
+// (64 VALU ops per candidate instead of 143). The K best approximate scores
+// are kept as 32-bit keys (FP16 score bits << 16 | scan index) by a
+// branch-free min/max cascade; those K candidates are then rescored exactly
+// in the reference's FP32 order and the exact top-8 is taken by
+// (score, scan index), which is the order the sequential strict-< insertion
+// of the reference produces. The result equals the FP32 path's unless an
+// exact top-8 candidate ranks below K approximately.
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ f16x2 pack_f16x2(float a, float b) {
+    return __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+
+__device__ __forceinline__ float dot2_acc(f16x2 a, f16x2 b, float c) {
+    return __builtin_amdgcn_fdot2(a, b, c, false);
+}
+
+template <int K>
+struct KeyList {
+    uint32_t k[K];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < K; ++i) k[i] = 0xFFFFFFFFu;
+    }
+    // keeps the K smallest keys, ascending
+    __device__ __forceinline__ void push(uint32_t key) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t lo = min(k[i], key);
+            key = max(k[i], key);
+            k[i] = lo;
+        }
+    }
+};
+
+// Exact top-8 by (error, scan index) for candidates arriving out of scan order.
+struct Best8Id {
+    Best8 b;
+    uint32_t id[8];
+    __device__ __forceinline__ void init(float err, uint32_t pos) {
+        b.init(err, pos);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) id[i] = 0;
+    }
+    __device__ __forceinline__ void insert(float err, uint32_t pos, uint32_t idn) {
+        bool c[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) c[i] = err < b.e[i] || (err == b.e[i] && idn < id[i]);
+#pragma unroll
+        for (int i = 7; i >= 1; --i) {
+            b.e[i] = c[i - 1] ? b.e[i - 1] : (c[i] ? err : b.e[i]);
+            b.p[i] = c[i - 1] ? b.p[i - 1] : (c[i] ? pos : b.p[i]);
+            id[i] = c[i - 1] ? id[i - 1] : (c[i] ? idn : id[i]);
+        }
+        b.e[0] = c[0] ? err : b.e[0];
+        b.p[0] = c[0] ? pos : b.p[0];
+        id[0] = c[0] ? idn : id[0];
+    }
+};
+
+// Advances (ph, colbase) to the next tile column.
+__device__ __forceinline__ void next_col(const TileGeom& g, int& ph, int& colbase) {
+    ++ph;
+    const bool wrap = ph == g.bs;
+    ph = wrap ? 0 : ph;
+    colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
+}
+
+// Exact distance of the candidate at (cx, cy), same op order as ring_ssd.
+__device__ __forceinline__ float exact_ssd_at(const float* __restrict__ lds, const TileGeom& g,
+                                              const float (&rb)[8][8], int cx, int cy) {
+    const int rt = cy - g.ty0;
+    const int tc = cx - g.tx0;
+    int q = tc / g.bs;
+    int ph = tc - q * g.bs;
+    int colbase = (ph * g.wq + q) * g.hp;
+    float s[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float* col = lds + colbase + rt;
+        float d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = rb[j][c] - col[j];
+        float e0 = d[0] * d[0];
+        e0 = fma_(d[2], d[2], e0);
+        e0 = fma_(d[4], d[4], e0);
+        e0 = fma_(d[6], d[6], e0);
+        float e1 = d[1] * d[1];
+        e1 = fma_(d[3], d[3], e1);
+        e1 = fma_(d[5], d[5], e1);
+        e1 = fma_(d[7], d[7], e1);
+        s[c] = e0 + e1;
+        next_col(g, ph, colbase);
+    }
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+template <int U, int K, bool kCount>
+__device__ __forceinline__ void scan_step_mixed(const float* __restrict__ lds, const TileGeom& g,
+                                                const f16x2 (&rbh)[8][4], f16x2 (&ring)[8][4],
+                                                int& ph, int& colbase, int rt, float m, KeyList<K>& kl,
+                                                int col, bool row_ok, int W, uint32_t idx, unsigned& cnt) {
+    const float* src = lds + colbase + rt;
+    float t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = src[j] - m;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) ring[(U + 7) & 7][qq] = pack_f16x2(t[2 * qq], t[2 * qq + 1]);
+    next_col(g, ph, colbase);
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int sl = (U + c) & 7;
+        const f16x2 d0 = rbh[c][0] - ring[sl][0];
+        const f16x2 d1 = rbh[c][1] - ring[sl][1];
+        const f16x2 d2 = rbh[c][2] - ring[sl][2];
+        const f16x2 d3 = rbh[c][3] - ring[sl][3];
+        a0 = dot2_acc(d0, d0, a0);
+        a1 = dot2_acc(d1, d1, a1);
+        a0 = dot2_acc(d2, d2, a0);
+        a1 = dot2_acc(d3, d3, a1);
+    }
+    const float approx = a0 + a1;
+    const bool ok = row_ok && col >= 0 && col <= W - 8;
+    if constexpr (kCount) cnt += ok ? 1u : 0u;
+    const uint32_t hb = (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)approx);
+    kl.push(ok ? ((hb << 16) | idx) : 0xFFFFFFFFu);
+}
+
+// Approximate scan of one window into kl. Scan index of (dyi, dxi):
+// idx_base + (dyi << dy_shift) + dxi, increasing in the reference's scan
+// order (window-major, then row-major).
+template <int K, bool kCount>
+__device__ __forceinline__ void scan_window_mixed(const float* __restrict__ lds, const TileGeom& g,
+                                                  const f16x2 (&rbh)[8][4], float m, KeyList<K>& kl,
+                                                  int cx, int cy, int range, int W, int H,
+                                                  uint32_t idx_base, int dy_shift, unsigned& cnt) {
+    const int n = 2 * range + 1;
+    f16x2 ring[8][4];
+    for (int dyi = 0; dyi < n; ++dyi) {
+        const int row = cy - range + dyi;
+        const bool row_ok = row >= 0 && row <= H - 8;
+        const int rt = row - g.ty0;
+        const int tc0 = cx - range - g.tx0;
+        int q = tc0 / g.bs;
+        int ph = tc0 - q * g.bs;
+        int colbase = (ph * g.wq + q) * g.hp;
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            const float* src = lds + colbase + rt;
+            float t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = src[j] - m;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) ring[s][qq] = pack_f16x2(t[2 * qq], t[2 * qq + 1]);
+            next_col(g, ph, colbase);
+        }
+        const int col0 = cx - range;
+        const uint32_t rowidx = idx_base + ((uint32_t)dyi << dy_shift);
+        for (int base = 0; base < n; base += 8) {
+#define BM3D_SCAN_STEP_MIXED(U)                                                                    \
+    if (base + U < n)                                                                              \
+        scan_step_mixed<U, K, kCount>(lds, g, rbh, ring, ph, colbase, rt, m, kl, col0 + base + U, \
+                                      row_ok, W, rowidx + (uint32_t)(base + U), cnt);
+            BM3D_SCAN_STEP_MIXED(0)
+            BM3D_SCAN_STEP_MIXED(1)
+            BM3D_SCAN_STEP_MIXED(2)
+            BM3D_SCAN_STEP_MIXED(3)
+            BM3D_SCAN_STEP_MIXED(4)
+            BM3D_SCAN_STEP_MIXED(5)
+            BM3D_SCAN_STEP_MIXED(6)
+            BM3D_SCAN_STEP_MIXED(7)
+#undef BM3D_SCAN_STEP_MIXED
+        }
+    }
+}
+
+// Exact rescoring of the K preselected keys into an exact top-8.
+// decode(idx) -> (cx, cy) of the candidate.
+template <int K, typename Decode>
+__device__ __forceinline__ void rescore(const float* __restrict__ lds, const TileGeom& g,
+                                        const float (&rb)[8][8], const KeyList<K>& kl, Best8Id& out,
+                                        int z, Decode decode) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t key = kl.k[i];
+        if (key != 0xFFFFFFFFu) {
+            const uint32_t idx = key & 0xFFFFu;
+            int cx, cy;
+            decode(idx, cx, cy);
+            const float e = exact_ssd_at(lds, g, rb, cx, cy);
+            out.insert(e, pack_pos(cx, cy, z), idx);
+        }
+    }
+}
+
 // Block matching: one lane per reference block; a work-group covers 16 x TBY
 // reference blocks (TBY = blockDim.x / 16). Grid: (ceil(nbx/16),
 // ceil(nby/TBY), frames). Spatial search on the centre slot (source/kernel.cu
 // :432-492), then for r > 0 the predictive temporal search
 // (source/kernel.cu:495-614, cpu_source/source.cpp:220-321), then the centre
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
-// with kCount, also adds the number of in-plane candidates to *a.ncand.
-template <bool kCount>
+// with kCount, also adds the number of in-plane candidates to *a.ncand;
+// with kMixed, uses the FP16 preselection + exact FP32 rescoring above.
+constexpr int kMixedK = 16;
+
+template <bool kCount, bool kMixed>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
     extern __shared__ float lds[];
     const int tby = blockDim.x >> 4;
@@ -340,8 +546,34 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
 
     unsigned cnt = 0;
     Best8 best;
-    best.init(kFltMax, pack_pos(x, y, r));
-    scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+    float m = 0.0f;
+    f16x2 rbh[8][4];
+    if constexpr (kMixed) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sum += rb[j][c];
+        m = sum * (1.0f / 64.0f);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) rbh[c][qq] = pack_f16x2(rb[2 * qq][c] - m, rb[2 * qq + 1][c] - m);
+        KeyList<kMixedK> kl;
+        kl.init();
+        scan_window_mixed<kMixedK, kCount>(lds, g, rbh, m, kl, x, y, a.bm_range, a.W, a.H, 0u, 7, cnt);
+        Best8Id bi;
+        bi.init(kFltMax, pack_pos(x, y, r));
+        const int R = a.bm_range;
+        rescore<kMixedK>(lds, g, rb, kl, bi, r, [&](uint32_t idx, int& cx, int& cy) {
+            cx = x - R + (int)(idx & 127u);
+            cy = y - R + (int)(idx >> 7);
+        });
+        best = bi.b;
+    } else {
+        best.init(kFltMax, pack_pos(x, y, r));
+        scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+    }
 
     if (r > 0) {
         uint32_t centre[8];
@@ -357,12 +589,36 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
                 stage_tile(lds, g, slot_plane(z), a.W, a.H, a.stride);
                 __syncthreads();
                 Best8 fr;
-                fr.init(kFltMax, pack_pos(0, 0, z));
-                for (int i = 0; i < a.ps_num; ++i) {
-                    uint32_t p = last[0];
+                if constexpr (kMixed) {
+                    KeyList<kMixedK> kl;
+                    kl.init();
+                    for (int i = 0; i < a.ps_num; ++i) {
+                        uint32_t p = last[0];
 #pragma unroll
-                    for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                    scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
+                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                        scan_window_mixed<kMixedK, kCount>(lds, g, rbh, m, kl, pos_x(p), pos_y(p), a.ps_range,
+                                                           a.W, a.H, (uint32_t)i << 12, 6, cnt);
+                    }
+                    Best8Id fi;
+                    fi.init(kFltMax, pack_pos(0, 0, z));
+                    const int ps = a.ps_range;
+                    rescore<kMixedK>(lds, g, rb, kl, fi, z, [&](uint32_t idx, int& cx, int& cy) {
+                        const uint32_t win = idx >> 12;
+                        uint32_t p = last[0];
+#pragma unroll
+                        for (int k = 1; k < 8; ++k) p = (win == (uint32_t)k) ? last[k] : p;
+                        cx = pos_x(p) - ps + (int)(idx & 63u);
+                        cy = pos_y(p) - ps + (int)((idx >> 6) & 63u);
+                    });
+                    fr = fi.b;
+                } else {
+                    fr.init(kFltMax, pack_pos(0, 0, z));
+                    for (int i = 0; i < a.ps_num; ++i) {
+                        uint32_t p = last[0];
+#pragma unroll
+                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                        scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
+                    }
                 }
                 for (int i = 0; i < a.ps_num; ++i) {
                     const float e = fr.err_at(i);
@@ -398,157 +654,238 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
 
 // ---------------------------------------------------------------- filtering
 
-// Transpose the 8x8 blocks of each group between "lane k holds column k"
-// (v[i][j]) and "lane t holds row t" (v[i][kk]) through LDS.
-__device__ __forceinline__ void transpose8(float (&v)[8][8], float* __restrict__ buf, int g, int k) {
-    float* gb = buf + g * kTransposeGroupStride;
-    __syncthreads();
+// Cross-lane moves inside 8-lane groups with DPP (VALU only, no LDS):
+// xor1 / xor2 are quad permutations, xor4 is a half-row mirror followed by a
+// quad reversal (7 - k, then 3 - (k mod 4) within the quad, gives k ^ 4).
+template <int kCtrl>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+}
+template <typename T> __device__ __forceinline__ T lane_xor1(T v) { return dpp_mov<0xB1>(v); }
+template <typename T> __device__ __forceinline__ T lane_xor2(T v) { return dpp_mov<0x4E>(v); }
+template <typename T> __device__ __forceinline__ T lane_xor4(T v) { return dpp_mov<0x1B>(dpp_mov<0x141>(v)); }
+
+// 8x8 transpose across the 8 lanes of a group: lane k holds x[j] = M[j][k]
+// before and x[j] = M[k][j] after. Three exchange stages (d = 4, 2, 1): the
+// element at (register j, lane k) trades places with (j ^ d, k ^ d) when bit
+// d of j and k differ.
+__device__ __forceinline__ void transpose_member(float (&x)[8], int k) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) {
+        if (j & 4) continue;
+        const float a = lane_xor4(x[j + 4]), b = lane_xor4(x[j]);
+        x[j] = (k & 4) ? a : x[j];
+        x[j + 4] = (k & 4) ? x[j + 4] : b;
+    }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) gb[i * 64 + j * 8 + k] = v[i][j];
-    __syncthreads();
+    for (int j = 0; j < 8; ++j) {
+        if (j & 2) continue;
+        const float a = lane_xor2(x[j + 2]), b = lane_xor2(x[j]);
+        x[j] = (k & 2) ? a : x[j];
+        x[j + 2] = (k & 2) ? x[j + 2] : b;
+    }
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) v[i][kk] = gb[i * 64 + k * 8 + kk];
+    for (int j = 0; j < 8; j += 2) {
+        const float a = lane_xor1(x[j + 1]), b = lane_xor1(x[j]);
+        x[j] = (k & 1) ? a : x[j];
+        x[j + 1] = (k & 1) ? x[j + 1] : b;
+    }
 }
 
-// Forward 3-D transform: along j (in lane, column layout), transpose, along
-// k and along i (in lane, row layout). Leaves the coefficients in row layout.
-__device__ __forceinline__ void forward3d(float (&v)[8][8], float* buf, int g, int k) {
+// Forward 3-D transform: per member along j (column layout), transpose,
+// along k (row layout); then along i. Coefficients end in row layout.
+__device__ __forceinline__ void forward3d(float (&v)[8][8], int k) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) dct8<false>(v[i]);
-    transpose8(v, buf, g, k);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dct8<false>(v[i]);
+    for (int i = 0; i < 8; ++i) {
+        dct8<false>(v[i]);
+        transpose_member(v[i], k);
+        dct8<false>(v[i]);
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c) dct8_strided<false>(v, c);
 }
 
-// Inverse 3-D transform from row layout: along i, along k, transpose, along j.
-__device__ __forceinline__ void inverse3d(float (&v)[8][8], float* buf, int g, int k) {
+// Inverse 3-D transform from row layout: along i; then per member along k,
+// transpose, along j. The estimate ends in column layout.
+__device__ __forceinline__ void inverse3d(float (&v)[8][8], int k) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) dct8_strided<true>(v, c);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) dct8<true>(v[i]);
-    transpose8(v, buf, g, k);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dct8<true>(v[i]);
+    for (int i = 0; i < 8; ++i) {
+        dct8<true>(v[i]);
+        transpose_member(v[i], k);
+        dct8<true>(v[i]);
+    }
 }
 
+// Collaborative filtering (source/kernel.cu:649-756). A work-group of
+// nwaves waves owns a tile of tbx x tby reference blocks and filters their
+// groups 8 per wave (lane k of a group holds column k; member transposes go
+// through DPP lane exchanges, not LDS). Weighted estimates are accumulated
+// as int32 fixed point (units of a.fx_unit) with integer LDS atomics -- LDS
+// float atomics are several times slower -- into a per-slot region covering
+// the tile plus `rm` pixels (every member of a tile's groups lies inside it
+// when rm >= bm_range + radius * ps_range); the region is then written to
+// scratch with plain stores and bm3d_finalize sums overlapping regions.
+// With extractor_exp the pre-rounded values are whole multiples of the unit,
+// so the integer sums are exact. Members outside the region go to global
+// float atomics on the stack. Member weights go to the corner map (one
+// atomic per member). Grid: (ntx, nty, frames).
 template <bool kFinal, bool kChroma>
-__global__ __launch_bounds__(64) void bm3d_filter(FilterArgs a) {
-    __shared__ float buf[8 * kTransposeGroupStride];
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
+    extern __shared__ float lds[];
+    const int tw = 2 * a.radius + 1;
+    const int region = tw * a.rh * a.rw;
+    int* acc = reinterpret_cast<int*>(lds);
+    const int nwaves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int g = lane >> 3;
     const int k = lane & 7;
-    const int f = blockIdx.y;
+    const int tx = blockIdx.x, ty = blockIdx.y, f = blockIdx.z;
     const int n = a.first_frame + f;
-    const int tw = 2 * a.radius + 1;
-    int blk = blockIdx.x * 8 + g;
-    const bool active = blk < a.nblocks;
-    blk = min(blk, a.nblocks - 1);
-
-    uint32_t pk[8];
-    {
-        const uint4* gp = reinterpret_cast<const uint4*>(a.groups + ((size_t)f * a.nblocks + blk) * 8);
-        uint4 q0 = gp[0], q1 = gp[1];
-        pk[0] = q0.x; pk[1] = q0.y; pk[2] = q0.z; pk[3] = q0.w;
-        pk[4] = q1.x; pk[5] = q1.y; pk[6] = q1.z; pk[7] = q1.w;
-    }
-    size_t off[8];  // element offset of member i (column k) within a plane-slot stack
-    int zz[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        // positions come from bm3d_match and are in range; the clamps only
-        // keep a corrupted list from addressing outside the planes
-        const int z = min(pos_z(pk[i]), 2 * a.radius);
-        const int px = min(pos_x(pk[i]), a.W - 8);
-        const int py = min(pos_y(pk[i]), a.H - 8);
-        pk[i] = pack_pos(px, py, z);
-        zz[i] = z;
-        const int fi = clampi(n - a.radius + z, 0, a.num_frames - 1);
-        off[i] = (size_t)fi * a.frame_stride + (size_t)py * a.stride + px + k;
-    }
-
+    const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
+    const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
+    const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
     const int nplanes = kChroma ? 3 : 1;
+
     for (int p = 0; p < nplanes; ++p) {
         const float sigma = a.sigma[p];
         if (kChroma && sigma < kFltEpsilon) continue;  // source/kernel.cu:657-668
+        for (int i = threadIdx.x; i < region; i += blockDim.x) acc[i] = 0;
+        __syncthreads();
         const float* src = a.src + (size_t)p * a.plane_stride;
-        float v[8][8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[i][j] = src[off[i] + (size_t)j * a.stride];
-        forward3d(v, buf, g, k);
+        float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
+        float* corner = a.corner + (size_t)f * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
 
-        float w;
-        if constexpr (kFinal) {
-            const float* ref = a.ref + (size_t)p * a.plane_stride;
-            float b[8][8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) b[i][j] = ref[off[i] + (size_t)j * a.stride];
-            forward3d(b, buf, g, k);
-            // Wiener shrinkage (source/kernel.cu:284-326); lane k now holds row k.
-            const float s2 = sigma * sigma;
-            float acc = 0.0f;
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const float sq = b[i][c] * b[i][c];
-                    float coef = sq / (sq + s2);
-                    if (i == 0 && c == 0) coef = (k == 0) ? 1.0f : coef;
-                    acc = fma_(coef, coef, acc);
-                    v[i][c] = (v[i][c] * coef) * (1.0f / 4096.0f);
-                }
-            acc = acc + __shfl_xor(acc, 1, 8);
-            acc = acc + __shfl_xor(acc, 2, 8);
-            acc = acc + __shfl_xor(acc, 4, 8);
-            w = 1.0f / acc;
-        } else {
-            // hard thresholding (source/kernel.cu:208-252)
-            int kept = 0;
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const float val = v[i][c];
-                    const float thr = (i == 0 && c == 0 && k == 0) ? 0.0f : sigma;
-                    const bool keep = fabsf(val) >= thr;
-                    kept += keep ? 1 : 0;
-                    v[i][c] = keep ? val * (1.0f / 4096.0f) : 0.0f;
-                }
-            kept += __shfl_xor(kept, 1, 8);
-            kept += __shfl_xor(kept, 2, 8);
-            kept += __shfl_xor(kept, 4, 8);
-            w = 1.0f / (float)kept;
-        }
-        inverse3d(v, buf, g, k);
+        for (int base = 0; base < ngroups; base += nwaves * 8) {
+            const int li = base + wave * 8 + g;
+            const int bx = tx * a.tbx + li % a.tbx;
+            const int by = ty * a.tby + li / a.tbx;
+            const bool active = li < ngroups && bx < a.nbx && by < a.nby;
+            const int blk = active ? by * a.nbx + bx : 0;
 
-        if (active) {
-            // aggregation with pre-rounding (source/kernel.cu:717-756)
-            float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
-            float* corner = a.corner + (size_t)f * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
-            const float wr = (w + E) - E;
+            uint32_t pk[8];
+            {
+                const uint4* gp = reinterpret_cast<const uint4*>(a.groups + ((size_t)f * a.nblocks + blk) * 8);
+                const uint4 q0 = gp[0], q1 = gp[1];
+                pk[0] = q0.x; pk[1] = q0.y; pk[2] = q0.z; pk[3] = q0.w;
+                pk[4] = q1.x; pk[5] = q1.y; pk[6] = q1.z; pk[7] = q1.w;
+            }
+            const float* mp[8];  // member block, column k, in the noisy clip
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const size_t o = (size_t)pos_y(pk[i]) * a.stride + pos_x(pk[i]) + k;
-                float* num = res + (size_t)(2 * zz[i]) * a.plane_size + o;
+                // positions come from bm3d_match and are in range; the clamps
+                // only keep a corrupted list from addressing outside the planes
+                const int z = min(pos_z(pk[i]), 2 * a.radius);
+                const int px = min(pos_x(pk[i]), a.W - 8);
+                const int py = min(pos_y(pk[i]), a.H - 8);
+                pk[i] = pack_pos(px, py, z);
+                const int fi = clampi(n - a.radius + z, 0, a.num_frames - 1);
+                mp[i] = src + (size_t)fi * a.frame_stride + (size_t)py * a.stride + px + k;
+            }
+            float v[8][8];
+            float w;
+            if constexpr (kFinal) {
+                // Wiener shrinkage (source/kernel.cu:284-326). The basic
+                // estimate's group is transformed and turned into coefficients
+                // before the noisy group is loaded, so at most two groups are
+                // live. Lane k holds row k of every member (row layout).
+                const ptrdiff_t roff = a.ref - a.src;  // ref shares the clip layout
+                float cf[8][8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    float val = w * v[i][j];
-                    val = (val + E) - E;
-                    unsafeAtomicAdd(num + (size_t)j * a.stride, val);
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) cf[i][j] = mp[i][roff + (ptrdiff_t)j * a.stride];
+                forward3d(cf, k);
+                const float s2 = sigma * sigma;
+                float sum = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const float sq = cf[i][c] * cf[i][c];
+                        float coef = sq / (sq + s2);
+                        if (i == 0 && c == 0) coef = (k == 0) ? 1.0f : coef;
+                        sum = fma_(coef, coef, sum);
+                        cf[i][c] = coef;
+                    }
+                sum = sum + lane_xor1(sum);
+                sum = sum + lane_xor2(sum);
+                sum = sum + lane_xor4(sum);
+                w = 1.0f / sum;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[i][j] = mp[i][(size_t)j * a.stride];
+                forward3d(v, k);
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) v[i][c] = (v[i][c] * cf[i][c]) * (1.0f / 4096.0f);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[i][j] = mp[i][(size_t)j * a.stride];
+                forward3d(v, k);
+                // hard thresholding (source/kernel.cu:208-252)
+                int kept = 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const float val = v[i][c];
+                        const float thr = (i == 0 && c == 0 && k == 0) ? 0.0f : sigma;
+                        const bool keep = fabsf(val) >= thr;
+                        kept += keep ? 1 : 0;
+                        v[i][c] = keep ? val * (1.0f / 4096.0f) : 0.0f;
+                    }
+                kept += lane_xor1(kept);
+                kept += lane_xor2(kept);
+                kept += lane_xor4(kept);
+                w = 1.0f / (float)kept;
+            }
+            inverse3d(v, k);
+
+            if (active) {
+                // aggregation with pre-rounding (source/kernel.cu:717-756)
+                const float wr = (w + E) - E;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int z = pos_z(pk[i]), px = pos_x(pk[i]), py = pos_y(pk[i]);
+                    const int lx = px - rx0, ly = py - ry0;
+                    if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
+                        int* q = acc + (z * a.rh + ly) * a.rw + lx + k;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            float val = w * v[i][j];
+                            val = (val + E) - E;
+                            atomicAdd(q + j * a.rw, (int)__builtin_rintf(val * a.fx_scale));
+                        }
+                    } else {
+                        float* q = res + (size_t)(2 * z) * a.plane_size + (size_t)py * a.stride + px + k;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            float val = w * v[i][j];
+                            val = (val + E) - E;
+                            unsafeAtomicAdd(q + (size_t)j * a.stride, val);
+                        }
+                    }
+                    if (k == 0)
+                        unsafeAtomicAdd(corner + (size_t)z * a.plane_size + (size_t)py * a.stride + px, wr);
                 }
-                if (k == 0) unsafeAtomicAdd(corner + (size_t)zz[i] * a.plane_size + o, wr);
             }
         }
+        __syncthreads();
+        int* out = a.scratch + ((((size_t)f * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
+        for (int i = threadIdx.x; i < region; i += blockDim.x) out[i] = acc[i];
+        __syncthreads();
     }
 }
 
@@ -563,6 +900,45 @@ __global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t inc = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += inc) q[i] = 0.0f;
+}
+
+// Weighted-sum rows of the stacks: the integer sum of the tile regions
+// bm3d_filter wrote to scratch that cover the pixel (at most 2 x 2 tiles),
+// converted once to float, plus the stack's own value when members could
+// fall outside the regions. Grid: (ceil(W/256), H, frames * planes * slots).
+__global__ void bm3d_finalize(FinalizeArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int fps = blockIdx.z;
+    if (x >= a.W) return;
+    const int z = fps % a.slots;
+    const int p = (fps / a.slots) % a.nplanes;
+    const int f = fps / (a.slots * a.nplanes);
+    float* num = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride +
+                 (size_t)(2 * z) * a.plane_size + (size_t)y * a.stride + x;
+    float s = a.accumulate ? num[0] : 0.0f;
+    long long isum = 0;
+    if (!(a.chroma && a.sigma[p] < kFltEpsilon)) {
+        const int region = a.slots * a.rh * a.rw;
+        const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
+        const int ylo_n = y - a.rm - 7 - (a.tby - 1) * a.bs;
+        const int xlo_n = x - a.rm - 7 - (a.tbx - 1) * a.bs;
+        const int ty_lo = max((ylo_n < 0 ? 0 : ylo_n / th) - 1, 0);
+        const int ty_hi = min((y + a.rm) / th + 1, a.nty - 1);
+        const int tx_lo = max((xlo_n < 0 ? 0 : xlo_n / tw_) - 1, 0);
+        const int tx_hi = min((x + a.rm) / tw_ + 1, a.ntx - 1);
+        for (int ty = ty_lo; ty <= ty_hi; ++ty) {
+            const int ly = y - (min(ty * th, a.H - 8) - a.rm);
+            if (ly < 0 || ly >= a.rh) continue;
+            for (int tx = tx_lo; tx <= tx_hi; ++tx) {
+                const int lx = x - (min(tx * tw_, a.W - 8) - a.rm);
+                if (lx < 0 || lx >= a.rw) continue;
+                const int* reg = a.scratch + ((((size_t)f * a.nty + ty) * a.ntx + tx) * a.nplanes + p) * region;
+                isum += reg[(z * a.rh + ly) * a.rw + lx];
+            }
+        }
+    }
+    num[0] = s + (float)isum * a.fx_unit;
 }
 
 // den[y][x] = sum over a, b in [0, 8) of corner[y - a][x - b]: each member
@@ -652,28 +1028,58 @@ __global__ void bm3d_vaggregate(VAggArgs a) {
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
-    if (a.ncand) hipLaunchKernelGGL(bm3d_match<true>, grid, dim3(16 * tby), lds_bytes, s, a);
-    else hipLaunchKernelGGL(bm3d_match<false>, grid, dim3(16 * tby), lds_bytes, s, a);
+    dim3 block(16 * tby);
+    if (a.mixed) {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match<true, true>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_match<false, true>), grid, block, lds_bytes, s, a);
+    } else {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match<true, false>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_match<false, false>), grid, block, lds_bytes, s, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t set_match_lds_limit(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bm3d_match<false>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&bm3d_match<true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<false, false>),
+                         reinterpret_cast<const void*>(&bm3d_match<true, false>),
+                         reinterpret_cast<const void*>(&bm3d_match<false, true>),
+                         reinterpret_cast<const void*>(&bm3d_match<true, true>)};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
-hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, hipStream_t s) {
-    dim3 grid((a.nblocks + 7) / 8, frames);
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
+                         hipStream_t s) {
+    dim3 grid(a.ntx, a.nty, frames);
+    dim3 block(64 * kFilterWaves);
     if (final_) {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((bm3d_filter<true, false>), grid, dim3(64), 0, s, a);
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<true, false>), grid, block, lds_bytes, s, a);
     } else {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter<false, true>), grid, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((bm3d_filter<false, false>), grid, dim3(64), 0, s, a);
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<false, true>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<false, false>), grid, block, lds_bytes, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t set_filter_lds_limit(size_t bytes) {
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_filter<true, true>),
+                         reinterpret_cast<const void*>(&bm3d_filter<true, false>),
+                         reinterpret_cast<const void*>(&bm3d_filter<false, true>),
+                         reinterpret_cast<const void*>(&bm3d_filter<false, false>)};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s) {
+    dim3 grid((a.W + 255) / 256, a.H, count);
+    hipLaunchKernelGGL(bm3d_finalize, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
