@@ -150,6 +150,84 @@ int bm3d_profile_read(int kernel, double *total_ms, long long *launches);
 int bm3d_count_candidates(bm3d_plan *plan, const bm3d_clip *src, const bm3d_clip *ref, int first_frame,
                           int count, unsigned long long *ncand, void *stream);
 
+/* ---- plugin-level filters ------------------------------------------------
+ * The three filters the bm3dcuda plugin registers (source/source.cpp:
+ * 991-1027), with its argument semantics: optional per-plane arrays whose
+ * missing entries repeat the previous one, the same defaults, checks and
+ * error messages (BM3DCreate, source/source.cpp:457-720; VAggregateCreate
+ * 880-909; BM3Dv2Create 911-989), and the same frame semantics (BM3DGetFrame
+ * 206-441, VAggregateGetFrame 746-862). Frames live in device memory; a
+ * get_frames call produces any range of output frames of a whole clip. */
+
+#define BM3D_GRAY 1
+#define BM3D_RGB 2
+#define BM3D_YUV 3
+
+typedef struct bm3d_video_info {
+    int color_family;    /* BM3D_GRAY, BM3D_RGB or BM3D_YUV */
+    int sample_type;     /* 0 integer, 1 float */
+    int bits_per_sample;
+    int subsampling_w, subsampling_h;
+    int num_planes;
+    int width, height;   /* 0 = variable (rejected, as in the reference) */
+    int num_frames;
+} bm3d_video_info;
+
+/* Arguments of BM3D / BM3Dv2 ("clip:clip;ref:clip:opt;sigma:float[]:opt;
+ * block_step:int[]:opt;bm_range:int[]:opt;radius:int:opt;ps_num:int[]:opt;
+ * ps_range:int[]:opt;chroma:int:opt;device_id:int:opt;fast:int:opt;
+ * extractor_exp:int:opt;zero_init:int:opt;", source/source.cpp:1001-1016).
+ * n_<name> is the number of values given (0 = key absent). `precision` has
+ * no counterpart in the reference (BM3D_PRECISION_*, default FP32). */
+typedef struct bm3d_args {
+    int n_sigma;          double sigma[3];
+    int n_block_step;     long long block_step[3];
+    int n_bm_range;       long long bm_range[3];
+    int n_radius;         long long radius;
+    int n_ps_num;         long long ps_num[3];
+    int n_ps_range;       long long ps_range[3];
+    int n_chroma;         long long chroma;
+    int n_device_id;      long long device_id;
+    int n_fast;           long long fast;
+    int n_extractor_exp;  long long extractor_exp;
+    int n_zero_init;      long long zero_init;
+    int n_precision;      long long precision;
+} bm3d_args;
+
+/* A clip's planes in device memory: frame f of plane p starts at
+ * data[p] + f * frame_stride[p]; rows are stride[p] floats apart. */
+typedef struct bm3d_frames {
+    float *data[3];
+    long long frame_stride[3];
+    int stride[3];
+} bm3d_frames;
+
+typedef struct bm3d_filter bm3d_filter;
+
+/* BM3D (BM3DCreate). ref: video info of "ref", or NULL when not given. */
+int bm3d_filter_create(const bm3d_args *args, const bm3d_video_info *clip, const bm3d_video_info *ref,
+                       bm3d_filter **filter);
+/* VAggregate (VAggregateCreate): clip = a BM3D output with radius > 0,
+ * src = the source clip, planes = indices of the planes to aggregate. */
+int bm3d_vaggregate_create(const long long *planes, int n_planes, const bm3d_video_info *clip,
+                           const bm3d_video_info *src, bm3d_filter **filter);
+/* BM3Dv2 (BM3Dv2Create): BM3D, then VAggregate when radius > 0; the source
+ * clip itself when no plane is processed. */
+int bm3dv2_filter_create(const bm3d_args *args, const bm3d_video_info *clip, const bm3d_video_info *ref,
+                         bm3d_filter **filter);
+
+/* Output video info (BM3DInit: height x 2(2r+1) for radius > 0), the
+ * temporal radius and the processed planes (the BM3D_V_radius and
+ * BM3D_V_process frame properties, source/source.cpp:428-435). */
+int bm3d_filter_info(const bm3d_filter *filter, bm3d_video_info *out, int *radius, int process[3]);
+
+/* Produces output frames first .. first+count-1. clip and ref (BM3D,
+ * BM3Dv2) or clip and src (VAggregate; pass src as `ref`) hold all frames
+ * of their clips; out receives frames at their own indices. */
+int bm3d_filter_get_frames(bm3d_filter *filter, const bm3d_frames *clip, const bm3d_frames *ref,
+                           const bm3d_frames *out, int first, int count, void *stream);
+void bm3d_filter_destroy(bm3d_filter *filter);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *bm3d_last_error(void);
 

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "bm3d_mi355x.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(bm3d_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(bm3d(?:v2)?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_header():

@@ -41,6 +41,33 @@ class Clip(ctypes.Structure):
                 ("num_frames", c_int)]
 
 
+class VideoInfo(ctypes.Structure):
+    _fields_ = [("color_family", c_int), ("sample_type", c_int), ("bits_per_sample", c_int),
+                ("subsampling_w", c_int), ("subsampling_h", c_int), ("num_planes", c_int),
+                ("width", c_int), ("height", c_int), ("num_frames", c_int)]
+
+
+class Args(ctypes.Structure):
+    _fields_ = [
+        ("n_sigma", c_int), ("sigma", ctypes.c_double * 3),
+        ("n_block_step", c_int), ("block_step", c_ll * 3),
+        ("n_bm_range", c_int), ("bm_range", c_ll * 3),
+        ("n_radius", c_int), ("radius", c_ll),
+        ("n_ps_num", c_int), ("ps_num", c_ll * 3),
+        ("n_ps_range", c_int), ("ps_range", c_ll * 3),
+        ("n_chroma", c_int), ("chroma", c_ll),
+        ("n_device_id", c_int), ("device_id", c_ll),
+        ("n_fast", c_int), ("fast", c_ll),
+        ("n_extractor_exp", c_int), ("extractor_exp", c_ll),
+        ("n_zero_init", c_int), ("zero_init", c_ll),
+        ("n_precision", c_int), ("precision", c_ll),
+    ]
+
+
+class Frames(ctypes.Structure):
+    _fields_ = [("data", c_void_p * 3), ("frame_stride", c_ll * 3), ("stride", c_int * 3)]
+
+
 class BM3DError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(message)
@@ -90,6 +117,21 @@ def lib():
     L.bm3d_count_candidates.argtypes = [P, ctypes.POINTER(Clip), ctypes.POINTER(Clip), c_int, c_int,
                                         ctypes.POINTER(ctypes.c_ulonglong), P]
     L.bm3d_count_candidates.restype = c_int
+    PV = ctypes.POINTER(VideoInfo)
+    PA = ctypes.POINTER(Args)
+    PF = ctypes.POINTER(Frames)
+    for name in ("bm3d_filter_create", "bm3dv2_filter_create"):
+        fn = getattr(L, name)
+        fn.argtypes = [PA, PV, PV, ctypes.POINTER(P)]
+        fn.restype = c_int
+    L.bm3d_vaggregate_create.argtypes = [ctypes.POINTER(c_ll), c_int, PV, PV, ctypes.POINTER(P)]
+    L.bm3d_vaggregate_create.restype = c_int
+    L.bm3d_filter_info.argtypes = [P, PV, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    L.bm3d_filter_info.restype = c_int
+    L.bm3d_filter_get_frames.argtypes = [P, PF, PF, PF, c_int, c_int, P]
+    L.bm3d_filter_get_frames.restype = c_int
+    L.bm3d_filter_destroy.argtypes = [P]
+    L.bm3d_filter_destroy.restype = None
     L.bm3d_last_error.argtypes = []
     L.bm3d_last_error.restype = ctypes.c_char_p
     L.bm3d_version.argtypes = []
@@ -111,6 +153,8 @@ EXPORTS = (
     "bm3d_execute_window", "bm3d_aggregate", "bm3d_vaggregate", "bm3d_match",
     "bm3d_num_blocks", "bm3d_last_error", "bm3d_version", "bm3d_profile_enable",
     "bm3d_profile_reset", "bm3d_profile_read", "bm3d_count_candidates",
+    "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
+    "bm3d_filter_get_frames", "bm3d_filter_destroy",
 )
 
 KERNEL_MATCH = 0

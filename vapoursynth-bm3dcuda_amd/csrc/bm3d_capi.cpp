@@ -17,17 +17,22 @@
 #include <string>
 #include <vector>
 
+#include "bm3d_internal.h"
 #include "bm3d_kernels.h"
 #include "bm3d_mi355x.h"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int bm3d::fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+namespace {
+
+using bm3d::fail;
 
 int hip_fail(hipError_t e, const char* what) {
     return fail(BM3D_EHIP, std::string("BM3D: '") + what + "' failed: " + hipGetErrorString(e));
