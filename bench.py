@@ -8,9 +8,10 @@ block_step=4, bm_range=9, ps_num=2, ps_range=4, radius=1:
     basic = BM3Dv2(src, radius=1)            -> BM3D stacks + VAggregate
     final = BM3Dv2(src, ref=basic, radius=1) -> BM3D stacks + VAggregate
 The clip is resident in HBM before the timed region; value = frames/s of
-the whole two-pass job over K steps, for the mixed-precision path (FP16
-block-matching preselection with exact FP32 rescoring). The plain FP32 path
-is timed the same way beside it (fp32_value, speedup_vs_fp32).
+the whole two-pass job over K steps, for the FP32 path (--precision, default
+fp32). The mixed-precision path (FP16 block-matching preselection with exact
+FP32 rescoring) is timed the same way beside it (mixed_value,
+mixed_speedup).
 
 roofline: the block-matching kernel (dominant) of the reported path.
 Algorithmic FLOPs per launch = 192 x (in-plane candidates evaluated in that
@@ -45,6 +46,19 @@ sys.path.insert(0, os.path.join(ROOT, "vapoursynth-bm3dcuda_amd"))
 H, W, N = 1080, 1920, 30
 PARAMS = dict(sigma=15.0, block_step=4, bm_range=9, radius=1, ps_num=2, ps_range=4)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak
+
+
+def profiled_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (lines `traffic <kernel> <precision> <bytes>` under profiles/),
+    or None when no summary covers this kernel and precision."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic" and f[1] == kernel and f[2] == precision:
+                return int(f[3])
+    return None
 
 
 def log(msg):
@@ -176,10 +190,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed"],
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"],
                     help="path reported as value (the other is timed beside it)")
     ap.add_argument("--frames", type=int, default=N)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the run, fill and copy a buffer of this many MiB (a known-size "
+                         "reference for HBM byte counters)")
     ap.add_argument("--no-compare", action="store_true", help="time only --precision")
     ap.add_argument("--cpu-threads", type=int, default=8)
     args = ap.parse_args()
@@ -248,7 +265,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(main_r["achieved_tflops"] / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("bm3d_match", args.precision),
             "flops_per_launch": main_r["flops_launch"],
             "avg_launch_ms": round(main_r["avg_launch_ms"], 4),
             "candidates_per_step": main_r["ncand"],
@@ -260,7 +277,8 @@ def main():
     }
     if "fp32" in res and "mixed" in res:
         out["fp32_value"] = round(res["fp32"]["fps"], 3)
-        out["speedup_vs_fp32"] = round(res["mixed"]["fps"] / res["fp32"]["fps"], 4)
+        out["mixed_value"] = round(res["mixed"]["fps"], 3)
+        out["mixed_speedup"] = round(res["mixed"]["fps"] / res["fp32"]["fps"], 4)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         bas = res[args.precision]["pipe"].basic.cpu().numpy()
@@ -268,6 +286,14 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+    if args.copy_mb > 0:
+        a = torch.empty(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        a.fill_(1.0)   # writes copy_mb MiB
+        b.copy_(a)     # reads and writes copy_mb MiB
+        torch.cuda.synchronize()
+        log(f"copy reference: fill + copy of {args.copy_mb} MiB done")
 
 
 if __name__ == "__main__":

@@ -135,6 +135,12 @@ int bm3d_match(bm3d_plan *plan, unsigned int *d_groups, const bm3d_clip *src, co
                int first_frame, int count, void *stream);
 long long bm3d_num_blocks(const bm3d_plan *plan);
 
+/* 1 when the plan matches with the FP16 MFMA pre-scoring kernel (precision
+ * MIXED and its work-group tile fits the device's LDS), 0 when it uses the
+ * FP32 kernel. Both give the same group lists. No counterpart in the
+ * reference. */
+int bm3d_plan_uses_mfma(const bm3d_plan *plan);
+
 /* Kernel timing (no counterpart in the reference). While enabled, every
  * launch of the block-matching (kernel 0) and filtering (kernel 1) kernels
  * is bracketed by HIP events on its stream. bm3d_profile_read waits for the

@@ -16,9 +16,6 @@ constexpr uint32_t kPosYMask = (1u << 13) - 1;
 constexpr int kMaxWidth = 1 << 14;
 constexpr int kMaxHeight = 1 << 13;
 constexpr int kMaxRadius = 15;
-// Limits of the mixed-precision matching path (scan indices fit 16 bits).
-constexpr int kMixedMaxRange = 63;
-constexpr int kMixedMaxPsRange = 31;
 
 // Waves per filter work-group (8 groups per wave).
 constexpr int kFilterWaves = 4;
@@ -35,7 +32,6 @@ struct MatchArgs {
     int tile_w, tile_h, tile_wq, tile_hp;
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
     unsigned long long* ncand;  // non-null: count in-plane candidates
-    int mixed;               // FP16 preselection + exact FP32 rescoring
 };
 
 struct FilterArgs {
@@ -109,8 +105,35 @@ struct VAggArgs {
     int radius, num_frames, first_frame;
 };
 
+// Block matching with FP16 MFMA pre-scoring and exact rescoring
+// (bm3d_match_mx.hip): 8 x 8 reference blocks per work-group, 16 per wave.
+constexpr int kMxWaves = 4;
+constexpr int kMxRefsX = 8, kMxRefsY = 8;
+constexpr int kMxRowBufWords = 2 * 8 * 21;  // per wave: 2 copies x 8 rows x 21 dwords
+constexpr int kMxMiscBytes = (4 * kMxWaves + 8 + kMxRefsX * kMxRefsY * 9) * 4;
+
+struct MxArgs {
+    const float* clip;       // plane used for matching, frame 0
+    long long frame_stride;
+    int num_frames, first_frame;
+    int W, H, stride;
+    int bs, bm_range, radius, ps_num, ps_range;
+    int nbx, nby, nblocks;
+    int rho;                 // bm_range + radius * ps_range: reach of a block's candidates
+    int t16_rows, t16_pitch; // FP16 tile: rows, halves per row (multiple of 8)
+    int nc_rows, nc_cols, nc_pitch;  // window sums: candidate rows, columns, floats per column
+    int hs_pitch;            // row sums (staging scratch over the maps): floats per column
+    int map_stride;          // halves of approximate-distance map per reference block
+    int win_stride;          // halves per temporal window in the map
+    unsigned off_nc, off_map, off_row, off_misc;  // LDS byte offsets
+    uint32_t* groups;
+    unsigned long long* ncand;
+};
+
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_lds_limit(size_t bytes);
+hipError_t launch_match_mx(const MxArgs& a, size_t lds_bytes, int frames, hipStream_t s);
+hipError_t set_match_mx_lds_limit(size_t bytes);
 hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
                          hipStream_t s);
 hipError_t set_filter_lds_limit(size_t bytes);

@@ -25,14 +25,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "bm3d_kernels.h"
+#include "bm3d_device.h"
 
 namespace bm3d {
 
 namespace {
 
-constexpr float kFltMax = 3.402823466e+38f;
-constexpr float kFltEpsilon = 1.192092896e-07f;
+using namespace dev;
 
 // Cosine constants of the 8-point DCT, written as double literals rounded
 // once to float (the oracle rounds the same literals the same way).
@@ -44,7 +43,6 @@ constexpr float kC3 = static_cast<float>(1.66293922460509004760575233432468773);
 constexpr float kC5 = static_cast<float>(1.11114046603920391150735136234616736);  // 2 cos(5pi/16)
 constexpr float kC7 = static_cast<float>(0.39018064403225654488047858302562963);  // 2 cos(7pi/16)
 
-__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // 8-point DCT-II scaled to 4 x orthonormal (the reference's normalisation,
 // cpu_source/source.cpp:444-536), by an even/odd split: a = x + rev(x) feeds
@@ -97,58 +95,7 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
         dct8_fwd(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c], v[5][c], v[6][c], v[7][c]);
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-__device__ __forceinline__ uint32_t pack_pos(int x, int y, int z) {
-    return (uint32_t)x | ((uint32_t)y << kPosYShift) | ((uint32_t)z << kPosZShift);
-}
-__device__ __forceinline__ int pos_x(uint32_t p) { return (int)(p & kPosXMask); }
-__device__ __forceinline__ int pos_y(uint32_t p) { return (int)((p >> kPosYShift) & kPosYMask); }
-__device__ __forceinline__ int pos_z(uint32_t p) { return (int)(p >> kPosZShift); }
-
 // ---------------------------------------------------------------- matching
-
-// Sorted (ascending, strict <) list of the 8 best candidates, kept in
-// registers. Equivalent to the shuffle-up insertion of source/kernel.cu:473-485
-// and the SIMD insertion of cpu_source/source.cpp:179-204.
-struct Best8 {
-    float e[8];
-    uint32_t p[8];
-
-    __device__ __forceinline__ void init(float err, uint32_t pos) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) { e[i] = err; p[i] = pos; }
-    }
-
-    __device__ __forceinline__ void insert(float err, uint32_t pos) {
-        bool c[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) c[i] = err < e[i];
-#pragma unroll
-        for (int i = 7; i >= 1; --i) {
-            float ne = c[i - 1] ? e[i - 1] : (c[i] ? err : e[i]);
-            uint32_t np = c[i - 1] ? p[i - 1] : (c[i] ? pos : p[i]);
-            e[i] = ne;
-            p[i] = np;
-        }
-        e[0] = c[0] ? err : e[0];
-        p[0] = c[0] ? pos : p[0];
-    }
-
-    // entry i for a wave-uniform runtime i
-    __device__ __forceinline__ float err_at(int i) const {
-        float r = e[0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k) r = (i == k) ? e[k] : r;
-        return r;
-    }
-    __device__ __forceinline__ uint32_t pos_at(int i) const {
-        uint32_t r = p[0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k) r = (i == k) ? p[k] : r;
-        return r;
-    }
-};
 
 // LDS tile: image rows [ty0, ty0 + th) x columns [tx0, tx0 + tw), stored as
 // `bs` column phases; phase ph holds columns ph, ph + bs, ... at consecutive
@@ -284,220 +231,14 @@ __device__ __forceinline__ void scan_window(const float* __restrict__ lds, const
     }
 }
 
-// ------------------------------------------------- mixed-precision matching
-//
-// Candidates are first scored approximately: the reference block and each
-// candidate column are offset by the reference block's mean and rounded to
-// FP16 (so flat regions keep their relative precision), differenced with
-// packed FP16 subtracts and squared-and-summed into FP32 with v_dot2_f32_f16
-// (64 VALU ops per candidate instead of 143). The K best approximate scores
-// are kept as 32-bit keys (FP16 score bits << 16 | scan index) by a
-// branch-free min/max cascade; those K candidates are then rescored exactly
-// in the reference's FP32 order and the exact top-8 is taken by
-// (score, scan index), which is the order the sequential strict-< insertion
-// of the reference produces. The result equals the FP32 path's unless an
-// exact top-8 candidate ranks below K approximately.
-
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ f16x2 pack_f16x2(float a, float b) {
-    return __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(a, b));
-}
-
-__device__ __forceinline__ float dot2_acc(f16x2 a, f16x2 b, float c) {
-    return __builtin_amdgcn_fdot2(a, b, c, false);
-}
-
-template <int K>
-struct KeyList {
-    uint32_t k[K];
-    __device__ __forceinline__ void init() {
-#pragma unroll
-        for (int i = 0; i < K; ++i) k[i] = 0xFFFFFFFFu;
-    }
-    // keeps the K smallest keys, ascending
-    __device__ __forceinline__ void push(uint32_t key) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const uint32_t lo = min(k[i], key);
-            key = max(k[i], key);
-            k[i] = lo;
-        }
-    }
-};
-
-// Exact top-8 by (error, scan index) for candidates arriving out of scan order.
-struct Best8Id {
-    Best8 b;
-    uint32_t id[8];
-    __device__ __forceinline__ void init(float err, uint32_t pos) {
-        b.init(err, pos);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) id[i] = 0;
-    }
-    __device__ __forceinline__ void insert(float err, uint32_t pos, uint32_t idn) {
-        bool c[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) c[i] = err < b.e[i] || (err == b.e[i] && idn < id[i]);
-#pragma unroll
-        for (int i = 7; i >= 1; --i) {
-            b.e[i] = c[i - 1] ? b.e[i - 1] : (c[i] ? err : b.e[i]);
-            b.p[i] = c[i - 1] ? b.p[i - 1] : (c[i] ? pos : b.p[i]);
-            id[i] = c[i - 1] ? id[i - 1] : (c[i] ? idn : id[i]);
-        }
-        b.e[0] = c[0] ? err : b.e[0];
-        b.p[0] = c[0] ? pos : b.p[0];
-        id[0] = c[0] ? idn : id[0];
-    }
-};
-
-// Advances (ph, colbase) to the next tile column.
-__device__ __forceinline__ void next_col(const TileGeom& g, int& ph, int& colbase) {
-    ++ph;
-    const bool wrap = ph == g.bs;
-    ph = wrap ? 0 : ph;
-    colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
-}
-
-// Exact distance of the candidate at (cx, cy), same op order as ring_ssd.
-__device__ __forceinline__ float exact_ssd_at(const float* __restrict__ lds, const TileGeom& g,
-                                              const float (&rb)[8][8], int cx, int cy) {
-    const int rt = cy - g.ty0;
-    const int tc = cx - g.tx0;
-    int q = tc / g.bs;
-    int ph = tc - q * g.bs;
-    int colbase = (ph * g.wq + q) * g.hp;
-    float s[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const float* col = lds + colbase + rt;
-        float d[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) d[j] = rb[j][c] - col[j];
-        float e0 = d[0] * d[0];
-        e0 = fma_(d[2], d[2], e0);
-        e0 = fma_(d[4], d[4], e0);
-        e0 = fma_(d[6], d[6], e0);
-        float e1 = d[1] * d[1];
-        e1 = fma_(d[3], d[3], e1);
-        e1 = fma_(d[5], d[5], e1);
-        e1 = fma_(d[7], d[7], e1);
-        s[c] = e0 + e1;
-        next_col(g, ph, colbase);
-    }
-    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-}
-
-template <int U, int K, bool kCount>
-__device__ __forceinline__ void scan_step_mixed(const float* __restrict__ lds, const TileGeom& g,
-                                                const f16x2 (&rbh)[8][4], f16x2 (&ring)[8][4],
-                                                int& ph, int& colbase, int rt, float m, KeyList<K>& kl,
-                                                int col, bool row_ok, int W, uint32_t idx, unsigned& cnt) {
-    const float* src = lds + colbase + rt;
-    float t[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) t[j] = src[j] - m;
-#pragma unroll
-    for (int qq = 0; qq < 4; ++qq) ring[(U + 7) & 7][qq] = pack_f16x2(t[2 * qq], t[2 * qq + 1]);
-    next_col(g, ph, colbase);
-    float a0 = 0.0f, a1 = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int sl = (U + c) & 7;
-        const f16x2 d0 = rbh[c][0] - ring[sl][0];
-        const f16x2 d1 = rbh[c][1] - ring[sl][1];
-        const f16x2 d2 = rbh[c][2] - ring[sl][2];
-        const f16x2 d3 = rbh[c][3] - ring[sl][3];
-        a0 = dot2_acc(d0, d0, a0);
-        a1 = dot2_acc(d1, d1, a1);
-        a0 = dot2_acc(d2, d2, a0);
-        a1 = dot2_acc(d3, d3, a1);
-    }
-    const float approx = a0 + a1;
-    const bool ok = row_ok && col >= 0 && col <= W - 8;
-    if constexpr (kCount) cnt += ok ? 1u : 0u;
-    const uint32_t hb = (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)approx);
-    kl.push(ok ? ((hb << 16) | idx) : 0xFFFFFFFFu);
-}
-
-// Approximate scan of one window into kl. Scan index of (dyi, dxi):
-// idx_base + (dyi << dy_shift) + dxi, increasing in the reference's scan
-// order (window-major, then row-major).
-template <int K, bool kCount>
-__device__ __forceinline__ void scan_window_mixed(const float* __restrict__ lds, const TileGeom& g,
-                                                  const f16x2 (&rbh)[8][4], float m, KeyList<K>& kl,
-                                                  int cx, int cy, int range, int W, int H,
-                                                  uint32_t idx_base, int dy_shift, unsigned& cnt) {
-    const int n = 2 * range + 1;
-    f16x2 ring[8][4];
-    for (int dyi = 0; dyi < n; ++dyi) {
-        const int row = cy - range + dyi;
-        const bool row_ok = row >= 0 && row <= H - 8;
-        const int rt = row - g.ty0;
-        const int tc0 = cx - range - g.tx0;
-        int q = tc0 / g.bs;
-        int ph = tc0 - q * g.bs;
-        int colbase = (ph * g.wq + q) * g.hp;
-#pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            const float* src = lds + colbase + rt;
-            float t[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = src[j] - m;
-#pragma unroll
-            for (int qq = 0; qq < 4; ++qq) ring[s][qq] = pack_f16x2(t[2 * qq], t[2 * qq + 1]);
-            next_col(g, ph, colbase);
-        }
-        const int col0 = cx - range;
-        const uint32_t rowidx = idx_base + ((uint32_t)dyi << dy_shift);
-        for (int base = 0; base < n; base += 8) {
-#define BM3D_SCAN_STEP_MIXED(U)                                                                    \
-    if (base + U < n)                                                                              \
-        scan_step_mixed<U, K, kCount>(lds, g, rbh, ring, ph, colbase, rt, m, kl, col0 + base + U, \
-                                      row_ok, W, rowidx + (uint32_t)(base + U), cnt);
-            BM3D_SCAN_STEP_MIXED(0)
-            BM3D_SCAN_STEP_MIXED(1)
-            BM3D_SCAN_STEP_MIXED(2)
-            BM3D_SCAN_STEP_MIXED(3)
-            BM3D_SCAN_STEP_MIXED(4)
-            BM3D_SCAN_STEP_MIXED(5)
-            BM3D_SCAN_STEP_MIXED(6)
-            BM3D_SCAN_STEP_MIXED(7)
-#undef BM3D_SCAN_STEP_MIXED
-        }
-    }
-}
-
-// Exact rescoring of the K preselected keys into an exact top-8.
-// decode(idx) -> (cx, cy) of the candidate.
-template <int K, typename Decode>
-__device__ __forceinline__ void rescore(const float* __restrict__ lds, const TileGeom& g,
-                                        const float (&rb)[8][8], const KeyList<K>& kl, Best8Id& out,
-                                        int z, Decode decode) {
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const uint32_t key = kl.k[i];
-        if (key != 0xFFFFFFFFu) {
-            const uint32_t idx = key & 0xFFFFu;
-            int cx, cy;
-            decode(idx, cx, cy);
-            const float e = exact_ssd_at(lds, g, rb, cx, cy);
-            out.insert(e, pack_pos(cx, cy, z), idx);
-        }
-    }
-}
-
 // Block matching: one lane per reference block; a work-group covers 16 x TBY
 // reference blocks (TBY = blockDim.x / 16). Grid: (ceil(nbx/16),
 // ceil(nby/TBY), frames). Spatial search on the centre slot (source/kernel.cu
 // :432-492), then for r > 0 the predictive temporal search
 // (source/kernel.cu:495-614, cpu_source/source.cpp:220-321), then the centre
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
-// with kCount, also adds the number of in-plane candidates to *a.ncand;
-// with kMixed, uses the FP16 preselection + exact FP32 rescoring above.
-constexpr int kMixedK = 16;
-
-template <bool kCount, bool kMixed>
+// with kCount, also adds the number of in-plane candidates to *a.ncand.
+template <bool kCount>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
     extern __shared__ float lds[];
     const int tby = blockDim.x >> 4;
@@ -546,34 +287,8 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
 
     unsigned cnt = 0;
     Best8 best;
-    float m = 0.0f;
-    f16x2 rbh[8][4];
-    if constexpr (kMixed) {
-        float sum = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) sum += rb[j][c];
-        m = sum * (1.0f / 64.0f);
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int qq = 0; qq < 4; ++qq) rbh[c][qq] = pack_f16x2(rb[2 * qq][c] - m, rb[2 * qq + 1][c] - m);
-        KeyList<kMixedK> kl;
-        kl.init();
-        scan_window_mixed<kMixedK, kCount>(lds, g, rbh, m, kl, x, y, a.bm_range, a.W, a.H, 0u, 7, cnt);
-        Best8Id bi;
-        bi.init(kFltMax, pack_pos(x, y, r));
-        const int R = a.bm_range;
-        rescore<kMixedK>(lds, g, rb, kl, bi, r, [&](uint32_t idx, int& cx, int& cy) {
-            cx = x - R + (int)(idx & 127u);
-            cy = y - R + (int)(idx >> 7);
-        });
-        best = bi.b;
-    } else {
-        best.init(kFltMax, pack_pos(x, y, r));
-        scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
-    }
+    best.init(kFltMax, pack_pos(x, y, r));
+    scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
 
     if (r > 0) {
         uint32_t centre[8];
@@ -589,36 +304,12 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
                 stage_tile(lds, g, slot_plane(z), a.W, a.H, a.stride);
                 __syncthreads();
                 Best8 fr;
-                if constexpr (kMixed) {
-                    KeyList<kMixedK> kl;
-                    kl.init();
-                    for (int i = 0; i < a.ps_num; ++i) {
-                        uint32_t p = last[0];
+                fr.init(kFltMax, pack_pos(0, 0, z));
+                for (int i = 0; i < a.ps_num; ++i) {
+                    uint32_t p = last[0];
 #pragma unroll
-                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                        scan_window_mixed<kMixedK, kCount>(lds, g, rbh, m, kl, pos_x(p), pos_y(p), a.ps_range,
-                                                           a.W, a.H, (uint32_t)i << 12, 6, cnt);
-                    }
-                    Best8Id fi;
-                    fi.init(kFltMax, pack_pos(0, 0, z));
-                    const int ps = a.ps_range;
-                    rescore<kMixedK>(lds, g, rb, kl, fi, z, [&](uint32_t idx, int& cx, int& cy) {
-                        const uint32_t win = idx >> 12;
-                        uint32_t p = last[0];
-#pragma unroll
-                        for (int k = 1; k < 8; ++k) p = (win == (uint32_t)k) ? last[k] : p;
-                        cx = pos_x(p) - ps + (int)(idx & 63u);
-                        cy = pos_y(p) - ps + (int)((idx >> 6) & 63u);
-                    });
-                    fr = fi.b;
-                } else {
-                    fr.init(kFltMax, pack_pos(0, 0, z));
-                    for (int i = 0; i < a.ps_num; ++i) {
-                        uint32_t p = last[0];
-#pragma unroll
-                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                        scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
-                    }
+                    for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                    scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
                 }
                 for (int i = 0; i < a.ps_num; ++i) {
                     const float e = fr.err_at(i);
@@ -653,21 +344,6 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
 }
 
 // ---------------------------------------------------------------- filtering
-
-// Cross-lane moves inside 8-lane groups with DPP (VALU only, no LDS):
-// xor1 / xor2 are quad permutations, xor4 is a half-row mirror followed by a
-// quad reversal (7 - k, then 3 - (k mod 4) within the quad, gives k ^ 4).
-template <int kCtrl>
-__device__ __forceinline__ float dpp_mov(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, 0xF, 0xF, false));
-}
-template <int kCtrl>
-__device__ __forceinline__ int dpp_mov(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
-}
-template <typename T> __device__ __forceinline__ T lane_xor1(T v) { return dpp_mov<0xB1>(v); }
-template <typename T> __device__ __forceinline__ T lane_xor2(T v) { return dpp_mov<0x4E>(v); }
-template <typename T> __device__ __forceinline__ T lane_xor4(T v) { return dpp_mov<0x1B>(dpp_mov<0x141>(v)); }
 
 // 8x8 transpose across the 8 lanes of a group: lane k holds x[j] = M[j][k]
 // before and x[j] = M[k][j] after. Three exchange stages (d = 4, 2, 1): the
@@ -1029,21 +705,14 @@ __global__ void bm3d_vaggregate(VAggArgs a) {
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
     dim3 block(16 * tby);
-    if (a.mixed) {
-        if (a.ncand) hipLaunchKernelGGL((bm3d_match<true, true>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_match<false, true>), grid, block, lds_bytes, s, a);
-    } else {
-        if (a.ncand) hipLaunchKernelGGL((bm3d_match<true, false>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_match<false, false>), grid, block, lds_bytes, s, a);
-    }
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match<true>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match<false>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
 
 hipError_t set_match_lds_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<false, false>),
-                         reinterpret_cast<const void*>(&bm3d_match<true, false>),
-                         reinterpret_cast<const void*>(&bm3d_match<false, true>),
-                         reinterpret_cast<const void*>(&bm3d_match<true, true>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<false>),
+                         reinterpret_cast<const void*>(&bm3d_match<true>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
