@@ -48,8 +48,8 @@ extern "C" {
 #define BM3D_EUNSUPPORTED 3 /* valid for the reference, outside this build  */
 
 #define BM3D_PRECISION_FP32 0  /* all FP32 on VALU                          */
-#define BM3D_PRECISION_MIXED 1 /* FP16 MFMA block-matching pre-selection,
-                                  exact FP32 rescoring                      */
+#define BM3D_PRECISION_MIXED 1 /* FP16 block-matching pre-scoring, exact
+                                  FP32 rescoring of the best 2k             */
 
 typedef struct bm3d_plan bm3d_plan;
 
@@ -135,11 +135,11 @@ int bm3d_match(bm3d_plan *plan, unsigned int *d_groups, const bm3d_clip *src, co
                int first_frame, int count, void *stream);
 long long bm3d_num_blocks(const bm3d_plan *plan);
 
-/* 1 when the plan matches with the FP16 MFMA pre-scoring kernel (precision
- * MIXED and its work-group tile fits the device's LDS), 0 when it uses the
- * FP32 kernel. Both give the same group lists. No counterpart in the
- * reference. */
-int bm3d_plan_uses_mfma(const bm3d_plan *plan);
+/* Block-matching kernel of the plan: 1 for the mixed-precision kernel
+ * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best 2k),
+ * 0 for the FP32 kernel (precision FP32, or a search window too large for
+ * the mixed kernel's 12-bit scan indices). No counterpart in the reference. */
+int bm3d_plan_matching_kernel(const bm3d_plan *plan);
 
 /* Kernel timing (no counterpart in the reference). While enabled, every
  * launch of the block-matching (kernel 0) and filtering (kernel 1) kernels

@@ -99,9 +99,9 @@ class Plan:
                                      int(first), int(count), _stream_ptr(stream)))
         return out
 
-    def uses_mfma(self):
-        """True when block matching runs on the FP16 MFMA pre-scoring kernel."""
-        return bool(C.lib().bm3d_plan_uses_mfma(self._h))
+    def uses_mixed_kernel(self):
+        """True when block matching runs on the mixed-precision kernel."""
+        return bool(C.lib().bm3d_plan_matching_kernel(self._h))
 
     def count_candidates(self, src, ref=None, first=0, count=None, stream=None):
         """Exact number of in-plane block-matching candidates for the frames."""

@@ -101,10 +101,10 @@ struct bm3d_plan {
     int* scratch = nullptr;
     float fx_unit = 0.0f;  // weighted-sum fixed-point unit
     int cap_frames = 0;
-    // MFMA matching kernel (precision MIXED), when its tile fits the LDS
-    bool use_mx = false;
-    bm3d::MxArgs mx{};
-    size_t mx_lds_bytes = 0;
+    // mixed-precision matching kernel (precision MIXED)
+    bool use_dx = false;
+    int tile_hp2 = 0;
+    size_t dx_lds_bytes = 0;
 };
 
 namespace {
@@ -128,53 +128,21 @@ void tile_geometry(bm3d_plan* p, int tby) {
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-// Geometry of the MFMA matching kernel (bm3d_match_mx.hip) for 8 x 8
-// reference blocks per work-group: the FP16 tile covers every candidate of
-// the work-group's blocks (reach bm_range + radius * ps_range) plus the 16
-// columns the 16 x 24 A-operand rows read past a window; the window-sum map
-// covers every candidate position. LDS reads past the end of the tile or the
-// map (rows and columns of MFMA blocks outside every window) stay inside
-// the allocation; the sizes below check it.
-void mx_geometry(bm3d_plan* p, int max_lds) {
+// Mixed-precision matching kernel (bm3d_match_dx.hip): the FP32 kernel's
+// tile, stored as two planes of FP16 pairs (hp2 words per column, odd so
+// that the 16 lanes of a row hit distinct banks). Scan indices are packed
+// in 12 bits.
+void dx_geometry(bm3d_plan* p, int max_lds) {
     const bm3d_desc& d = p->d;
-    p->use_mx = false;
+    p->use_dx = false;
     if (d.precision != BM3D_PRECISION_MIXED) return;
     const int nWs = 2 * d.bm_range + 1, nWt = 2 * d.ps_range + 1;
-    if (nWs > 63 || nWt > 63) return;  // scan positions are packed in 6 bits
-    bm3d::MxArgs& a = p->mx;
-    a = bm3d::MxArgs{};
-    a.rho = p->margin;
-    const int span_x = (bm3d::kMxRefsX - 1) * d.block_step + 8 + 2 * a.rho;
-    const int span_y = (bm3d::kMxRefsY - 1) * d.block_step + 8 + 2 * a.rho;
-    a.t16_pitch = round_up(span_x + 16, 8);
-    a.t16_rows = span_y;
-    a.nc_cols = span_x - 7;
-    a.nc_rows = span_y - 7;
-    a.nc_pitch = round_up(a.nc_rows, 4);
-    a.win_stride = round_up(nWt * nWt, 4);
-    a.map_stride = round_up(nWs * nWs, 4);
-    if (d.radius > 0) a.map_stride = std::max(a.map_stride, d.ps_num * a.win_stride);
-    const size_t t16_bytes = (size_t)a.t16_rows * a.t16_pitch * 2;
-    const size_t nc_bytes = (size_t)a.nc_cols * a.nc_pitch * 4;
-    a.hs_pitch = round_up(a.t16_rows, 4);
-    // the maps region also holds the row sums while a slot is staged (the
-    // vertical sums read up to 24 floats past a column)
-    const size_t map_bytes = std::max((size_t)bm3d::kMxRefsX * bm3d::kMxRefsY * a.map_stride * 2,
-                                      ((size_t)a.nc_cols * a.hs_pitch + 24) * 4);
-    a.off_nc = (unsigned)round_up((int)t16_bytes, 16);
-    a.off_map = a.off_nc + (unsigned)round_up((int)nc_bytes, 16);
-    a.off_row = a.off_map + (unsigned)round_up((int)map_bytes, 16);
-    a.off_misc = a.off_row + (unsigned)(bm3d::kMxWaves * bm3d::kMxRowBufWords * 4);
-    size_t total = a.off_misc + (size_t)bm3d::kMxMiscBytes;
-    // furthest reads: A operand row span_y + 14, column span_x + 15 (+8
-    // halves); window sums column span_x + 7, row span_y + 7 (+4 floats)
-    const size_t t16_end = ((size_t)(span_y + 14) * a.t16_pitch + span_x + 16) * 2;
-    const size_t nc_end = a.off_nc + ((size_t)(span_x + 7) * a.nc_pitch + span_y + 8) * 4;
-    total = std::max(total, std::max(t16_end, nc_end));
-    total = (size_t)round_up((int)total, 16);
-    if (total > (size_t)max_lds) return;
-    p->mx_lds_bytes = total;
-    p->use_mx = true;
+    if (nWs * nWs > 4096 || (d.radius > 0 && d.ps_num * nWt * nWt > 4096)) return;
+    p->tile_hp2 = ((p->tile_h + 1) / 2 + 1) | 1;
+    const size_t bytes = (size_t)2 * d.block_step * p->tile_wq * p->tile_hp2 * 4;
+    if (bytes > (size_t)max_lds) return;
+    p->dx_lds_bytes = bytes;
+    p->use_dx = true;
 }
 
 // LDS bytes of the filter kernel for a tile of tbx x tby blocks and margin rm.
@@ -224,27 +192,13 @@ int ensure_scratch(bm3d_plan* p, int frames) {
 }
 
 // Block matching for frames first .. first + count - 1 into `groups` (or
-// counting candidates into `ncand`): the MFMA kernel when the plan selected
-// it, else the FP32 kernel. Both produce the same lists.
+// counting candidates into `ncand`): the mixed-precision kernel when the
+// plan selected it, else the FP32 kernel.
 hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long long* ncand,
                            const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
                            hipStream_t stream) {
     const bm3d_desc& d = p->d;
     const float* clip = d.final_ ? ref->data : src->data;  // matching runs on ref in the final pass
-    if (p->use_mx) {
-        bm3d::MxArgs a = p->mx;
-        a.clip = clip;
-        a.frame_stride = src->frame_stride;
-        a.num_frames = src->num_frames;
-        a.first_frame = first_frame;
-        a.W = d.width; a.H = d.height; a.stride = d.stride;
-        a.bs = d.block_step; a.bm_range = d.bm_range; a.radius = d.radius;
-        a.ps_num = d.ps_num; a.ps_range = d.ps_range;
-        a.nbx = p->nbx; a.nby = p->nby; a.nblocks = p->nblocks;
-        a.groups = groups;
-        a.ncand = ncand;
-        return bm3d::launch_match_mx(a, p->mx_lds_bytes, count, stream);
-    }
     bm3d::MatchArgs m{};
     m.clip = clip;
     m.frame_stride = src->frame_stride;
@@ -258,6 +212,10 @@ hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
     m.groups = groups;
     m.ncand = ncand;
+    if (p->use_dx) {
+        m.tile_hp2 = p->tile_hp2;
+        return bm3d::launch_match_dx(m, p->tby, p->dx_lds_bytes, count, stream);
+    }
     return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
 }
 
@@ -353,9 +311,9 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
             return hip_fail(e, "hipFuncSetAttribute");
         }
     }
-    mx_geometry(p, max_lds);
-    if (p->use_mx && p->mx_lds_bytes > 65536) {
-        e = bm3d::set_match_mx_lds_limit(p->mx_lds_bytes);
+    dx_geometry(p, max_lds);
+    if (p->use_dx && p->dx_lds_bytes > 65536) {
+        e = bm3d::set_match_dx_lds_limit(p->dx_lds_bytes);
         if (e != hipSuccess) {
             delete p;
             return hip_fail(e, "hipFuncSetAttribute");
@@ -526,7 +484,7 @@ int bm3d_profile_read(int kernel, double* total_ms, long long* launches) {
     return BM3D_OK;
 }
 
-int bm3d_plan_uses_mfma(const bm3d_plan* p) { return p && p->use_mx ? 1 : 0; }
+int bm3d_plan_matching_kernel(const bm3d_plan* p) { return p && p->use_dx ? 1 : 0; }
 
 int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
                           unsigned long long* ncand, void* stream) {

@@ -30,6 +30,7 @@ struct MatchArgs {
     int nbx, nby, nblocks;
     int margin;              // bm_range + radius * ps_range
     int tile_w, tile_h, tile_wq, tile_hp;
+    int tile_hp2;            // mixed kernel: FP16 pair words per tile column
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
     unsigned long long* ncand;  // non-null: count in-plane candidates
 };
@@ -105,35 +106,14 @@ struct VAggArgs {
     int radius, num_frames, first_frame;
 };
 
-// Block matching with FP16 MFMA pre-scoring and exact rescoring
-// (bm3d_match_mx.hip): 8 x 8 reference blocks per work-group, 16 per wave.
-constexpr int kMxWaves = 4;
-constexpr int kMxRefsX = 8, kMxRefsY = 8;
-constexpr int kMxRowBufWords = 2 * 8 * 21;  // per wave: 2 copies x 8 rows x 21 dwords
-constexpr int kMxMiscBytes = (4 * kMxWaves + 8 + kMxRefsX * kMxRefsY * 9) * 4;
-
-struct MxArgs {
-    const float* clip;       // plane used for matching, frame 0
-    long long frame_stride;
-    int num_frames, first_frame;
-    int W, H, stride;
-    int bs, bm_range, radius, ps_num, ps_range;
-    int nbx, nby, nblocks;
-    int rho;                 // bm_range + radius * ps_range: reach of a block's candidates
-    int t16_rows, t16_pitch; // FP16 tile: rows, halves per row (multiple of 8)
-    int nc_rows, nc_cols, nc_pitch;  // window sums: candidate rows, columns, floats per column
-    int hs_pitch;            // row sums (staging scratch over the maps): floats per column
-    int map_stride;          // halves of approximate-distance map per reference block
-    int win_stride;          // halves per temporal window in the map
-    unsigned off_nc, off_map, off_row, off_misc;  // LDS byte offsets
-    uint32_t* groups;
-    unsigned long long* ncand;
-};
-
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_lds_limit(size_t bytes);
-hipError_t launch_match_mx(const MxArgs& a, size_t lds_bytes, int frames, hipStream_t s);
-hipError_t set_match_mx_lds_limit(size_t bytes);
+// Mixed-precision matching (bm3d_match_dx.hip): same launch geometry as
+// launch_match; LDS bytes as returned by dx_lds_bytes.
+hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
+hipError_t set_match_dx_lds_limit(size_t bytes);
+// Keys kept per temporal frame (at least 2 ps_num).
+int dx_temporal_keys(int ps_num);
 hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
                          hipStream_t s);
 hipError_t set_filter_lds_limit(size_t bytes);

@@ -25,7 +25,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "bm3d_device.h"
+#include "bm3d_scan.h"
 
 namespace bm3d {
 
@@ -96,140 +96,6 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
 }
 
 // ---------------------------------------------------------------- matching
-
-// LDS tile: image rows [ty0, ty0 + th) x columns [tx0, tx0 + tw), stored as
-// `bs` column phases; phase ph holds columns ph, ph + bs, ... at consecutive
-// column slots, and each column is `hp` consecutive words (rows). Lanes of a
-// work-group row hold blocks bs columns apart, so they read the same phase
-// at consecutive column slots.
-struct TileGeom {
-    int tx0, ty0;  // image coordinates of the tile origin (may be negative)
-    int tw, th;    // tile extent
-    int bs;        // number of column phases
-    int wq;        // column slots per phase
-    int hp;        // words per column (>= th)
-};
-
-__device__ __forceinline__ int tile_col_base(const TileGeom& g, int tc) {
-    int q = tc / g.bs;
-    int ph = tc - q * g.bs;
-    return (ph * g.wq + q) * g.hp;
-}
-
-// Fill the tile from one plane; out-of-image pixels replicate the border
-// (they are only ever read for candidates that are masked off).
-__device__ __forceinline__ void stage_tile(float* __restrict__ lds, const TileGeom& g,
-                           const float* __restrict__ img, int W, int H, int stride) {
-    const int n = g.tw * g.th;
-    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        int r = idx / g.tw;
-        int c = idx - r * g.tw;
-        int gx = clampi(g.tx0 + c, 0, W - 1);
-        int gy = clampi(g.ty0 + r, 0, H - 1);
-        lds[tile_col_base(g, c) + r] = img[(size_t)gy * stride + gx];
-    }
-}
-
-// Load one 8-row column of the tile into a ring slot.
-__device__ __forceinline__ void load_col(float (&dst)[8], const float* __restrict__ src) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) dst[j] = src[j];
-}
-
-// Distance in the exact order of cpu_source/source.cpp:123-136 + 98-103:
-// per column an even-row and an odd-row FMA chain, then a pairwise tree.
-// ring slot of candidate column c is (u + c) & 7.
-template <int U>
-__device__ __forceinline__ float ring_ssd(const float (&rb)[8][8], const float (&ring)[8][8]) {
-    float s[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int sl = (U + c) & 7;
-        float d0 = rb[0][c] - ring[sl][0];
-        float d1 = rb[1][c] - ring[sl][1];
-        float d2 = rb[2][c] - ring[sl][2];
-        float d3 = rb[3][c] - ring[sl][3];
-        float d4 = rb[4][c] - ring[sl][4];
-        float d5 = rb[5][c] - ring[sl][5];
-        float d6 = rb[6][c] - ring[sl][6];
-        float d7 = rb[7][c] - ring[sl][7];
-        float e0 = d0 * d0;
-        e0 = fma_(d2, d2, e0);
-        e0 = fma_(d4, d4, e0);
-        e0 = fma_(d6, d6, e0);
-        float e1 = d1 * d1;
-        e1 = fma_(d3, d3, e1);
-        e1 = fma_(d5, d5, e1);
-        e1 = fma_(d7, d7, e1);
-        s[c] = e0 + e1;
-    }
-    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-}
-
-template <int U, bool kCount>
-__device__ __forceinline__ void scan_step(const float* __restrict__ lds, const TileGeom& g,
-                                          const float (&rb)[8][8], float (&ring)[8][8],
-                                          int& ph, int& colbase, int rt, Best8& best,
-                                          int col, int row, bool row_ok, int z, int W,
-                                          unsigned& cnt) {
-    load_col(ring[(U + 7) & 7], lds + colbase + rt);
-    // advance to the next tile column
-    ++ph;
-    const bool wrap = ph == g.bs;
-    ph = wrap ? 0 : ph;
-    colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
-    float e = ring_ssd<U>(rb, ring);
-    const bool ok = row_ok && col >= 0 && col <= W - 8;
-    if constexpr (kCount) cnt += ok ? 1u : 0u;
-    if (ok && e < best.e[7]) best.insert(e, pack_pos(col, row, z));
-}
-
-// Exhaustive scan of the window of radius `range` around (cx, cy), clamped
-// to the plane, row-major (cpu_source/source.cpp:143-215). All lanes walk
-// the full (2 range + 1)^2 square so the loop is wave-uniform; candidates
-// outside the plane are masked, which leaves the scan order of the valid
-// ones -- and hence tie-breaking -- unchanged.
-template <bool kCount>
-__device__ __forceinline__ void scan_window(const float* __restrict__ lds, const TileGeom& g,
-                                            const float (&rb)[8][8], Best8& best,
-                                            int cx, int cy, int range, int z, int W, int H,
-                                            unsigned& cnt) {
-    const int n = 2 * range + 1;
-    float ring[8][8];
-    for (int dyi = 0; dyi < n; ++dyi) {
-        const int row = cy - range + dyi;
-        const bool row_ok = row >= 0 && row <= H - 8;
-        const int rt = row - g.ty0;
-        const int tc0 = cx - range - g.tx0;
-        int q = tc0 / g.bs;
-        int ph = tc0 - q * g.bs;
-        int colbase = (ph * g.wq + q) * g.hp;
-        // preload the first 7 columns
-#pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            load_col(ring[s], lds + colbase + rt);
-            ++ph;
-            const bool wrap = ph == g.bs;
-            ph = wrap ? 0 : ph;
-            colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
-        }
-        const int col0 = cx - range;
-        for (int base = 0; base < n; base += 8) {
-#define BM3D_SCAN_STEP(U)                                                                    \
-    if (base + U < n)                                                                        \
-        scan_step<U, kCount>(lds, g, rb, ring, ph, colbase, rt, best, col0 + base + U, row, row_ok, z, W, cnt);
-            BM3D_SCAN_STEP(0)
-            BM3D_SCAN_STEP(1)
-            BM3D_SCAN_STEP(2)
-            BM3D_SCAN_STEP(3)
-            BM3D_SCAN_STEP(4)
-            BM3D_SCAN_STEP(5)
-            BM3D_SCAN_STEP(6)
-            BM3D_SCAN_STEP(7)
-#undef BM3D_SCAN_STEP
-        }
-    }
-}
 
 // Block matching: one lane per reference block; a work-group covers 16 x TBY
 // reference blocks (TBY = blockDim.x / 16). Grid: (ceil(nbx/16),

@@ -1,0 +1,519 @@
+// Mixed-precision block matching (BM3D_PRECISION_MIXED).
+//
+// The search of source/kernel.cu:432-647 (cpu_source/source.cpp:143-321)
+// with the candidate distances pre-scored in FP16 and only the best 2k
+// rescored exactly in the reference's FP32 order (k = 8 for the spatial
+// search, k = ps_num for each temporal frame: only the first ps_num entries
+// of a frame's list are merged into the group and centre the next frame's
+// windows, source/kernel.cu:568-611).
+//
+// Layout as in the FP32 kernel: one lane per reference block, 16 x TBY
+// blocks per work-group, the work-group's search windows staged in LDS one
+// slot at a time, each lane sliding an 8-column ring across a window. The
+// tile is stored as FP16 pairs of vertically adjacent pixels minus a tile
+// offset mu (two copies, pairs starting on even and on odd rows), so a
+// candidate column of 8 rows is 4 packed registers. Per candidate:
+//
+//   D  = <r', c'>            32 v_dot2c_f32_f16 (FP32 accumulation)
+//   N  = |c'|^2              sliding sum of per-column v_dot2c sums
+//   A  = |r'|^2 + N - 2 D    the squared distance up to FP16 rounding
+//
+// A is packed with the candidate's scan index into a 32-bit key (A's
+// exponent and 11 mantissa bits above a 12-bit index) and kept by a
+// branch-free min/max cascade of the 2k smallest keys. The kept candidates
+// are rescored exactly and ordered by (distance, scan index), which is the
+// order the reference's strict sorted insertion in scan order produces; the
+// list equals the FP32 path's unless an exact top-k candidate ranks below 2k
+// approximately. A slot whose pixels are not finite or span more than +-8
+// around mu is searched exhaustively in FP32, exactly as the FP32 kernel.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bm3d_scan.h"
+
+namespace bm3d {
+
+namespace {
+
+using namespace dev;
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr float kDxMaxSpan = 8.0f;
+constexpr uint32_t kIdxBits = 12;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1;
+constexpr int kKSpatial = 16;  // 2 x 8
+
+__device__ __forceinline__ float dot2(f16x2 a, f16x2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+__device__ __forceinline__ f16x2 as_h2(uint32_t u) { return __builtin_bit_cast(f16x2, u); }
+
+// The K smallest 32-bit keys, ascending (branch-free min/max cascade).
+template <int K>
+struct KeyList {
+    uint32_t k[K];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < K; ++i) k[i] = 0xFFFFFFFFu;
+    }
+    __device__ __forceinline__ void push(uint32_t key) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t lo = min(k[i], key);
+            key = max(k[i], key);
+            k[i] = lo;
+        }
+    }
+    // removes and returns the smallest key
+    __device__ __forceinline__ uint32_t pop() {
+        const uint32_t r = k[0];
+#pragma unroll
+        for (int i = 0; i + 1 < K; ++i) k[i] = k[i + 1];
+        k[K - 1] = 0xFFFFFFFFu;
+        return r;
+    }
+};
+
+// Exact top 8 by (distance, scan index) for candidates arriving out of scan
+// order (the reference's list restricted to them).
+struct Best8Id {
+    Best8 b;
+    uint32_t id[8];
+    __device__ __forceinline__ void init(float err, uint32_t pos) {
+        b.init(err, pos);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) id[i] = 0xFFFFFFFFu;
+    }
+    __device__ __forceinline__ void insert(float err, uint32_t pos, uint32_t idn) {
+        bool c[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) c[i] = err < b.e[i] || (err == b.e[i] && idn < id[i]);
+#pragma unroll
+        for (int i = 7; i >= 1; --i) {
+            b.e[i] = c[i - 1] ? b.e[i - 1] : (c[i] ? err : b.e[i]);
+            b.p[i] = c[i - 1] ? b.p[i - 1] : (c[i] ? pos : b.p[i]);
+            id[i] = c[i - 1] ? id[i - 1] : (c[i] ? idn : id[i]);
+        }
+        b.e[0] = c[0] ? err : b.e[0];
+        b.p[0] = c[0] ? pos : b.p[0];
+        id[0] = c[0] ? idn : id[0];
+    }
+};
+
+// FP16 pair tile: two planes of `plane` dwords; in plane par, column tc's
+// word i (at the FP32 kernel's column-phase position, hp2 words per column)
+// holds rows (2 i + par, 2 i + par + 1) minus mu.
+struct PairGeom {
+    int tx0, ty0, tw, th, bs, wq, hp2, plane;
+};
+
+__device__ __forceinline__ int pair_col_base(const PairGeom& g, int tc) {
+    const int q = tc / g.bs;
+    const int ph = tc - q * g.bs;
+    return (ph * g.wq + q) * g.hp2;
+}
+
+__device__ __forceinline__ void next_pair_col(const PairGeom& g, int& ph, int& colbase) {
+    ++ph;
+    const bool wrap = ph == g.bs;
+    ph = wrap ? 0 : ph;
+    colbase += wrap ? (g.hp2 - (g.bs - 1) * g.wq * g.hp2) : g.wq * g.hp2;
+}
+
+// Stages the FP16 pairs; returns true when every staged pixel is within
+// kDxMaxSpan of mu (finite).
+__device__ bool stage_pairs(uint32_t* __restrict__ lds, const PairGeom& g, const float* __restrict__ img,
+                            int W, int H, int stride, float mu) {
+    const int n = g.tw * g.hp2;
+    bool ok = true;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+        const int c = (int)((unsigned)idx / (unsigned)g.hp2);
+        const int i = idx - c * g.hp2;
+        const int gx = clampi(g.tx0 + c, 0, W - 1);
+        const float* col = img + gx;
+        auto px = [&](int r) -> _Float16 {
+            const int gy = clampi(g.ty0 + min(r, g.th - 1), 0, H - 1);
+            return (_Float16)(col[(size_t)gy * stride] - mu);
+        };
+        const _Float16 p0 = px(2 * i), p1 = px(2 * i + 1), p2 = px(2 * i + 2);
+        ok &= __builtin_fabsf((float)p0) <= kDxMaxSpan && __builtin_fabsf((float)p1) <= kDxMaxSpan &&
+              __builtin_fabsf((float)p2) <= kDxMaxSpan;
+        const int base = pair_col_base(g, c) + i;
+        lds[base] = __builtin_bit_cast(uint32_t, f16x2{p0, p1});
+        lds[g.plane + base] = __builtin_bit_cast(uint32_t, f16x2{p1, p2});
+    }
+    return __syncthreads_and(ok);
+}
+
+// Sum of squares of a packed column (exact FP16 products, FP32 sums).
+__device__ __forceinline__ float col_sq(const f16x2 (&c)[4]) {
+    return dot2(c[3], c[3], dot2(c[2], c[2], dot2(c[1], c[1], dot2(c[0], c[0], 0.0f))));
+}
+
+__device__ __forceinline__ void load_pair_col(f16x2 (&dst)[4], const uint32_t* __restrict__ src) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dst[i] = as_h2(src[i]);
+}
+
+template <int U, int K, bool kCount>
+__device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const PairGeom& g,
+                                        const f16x2 (&rp)[8][4], float nr, f16x2 (&ring)[8][4],
+                                        float (&csq)[8], float& N, int& ph, int& colbase, int off,
+                                        KeyList<K>& kl, int col, bool row_ok, int W, uint32_t idx,
+                                        unsigned& cnt) {
+    load_pair_col(ring[(U + 7) & 7], lds + off + colbase);
+    next_pair_col(g, ph, colbase);
+    csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
+    N += csq[(U + 7) & 7];
+    float d0 = 0.0f, d1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c += 2) {
+        const int s0 = (U + c) & 7, s1 = (U + c + 1) & 7;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            d0 = dot2(rp[c][i], ring[s0][i], d0);
+            d1 = dot2(rp[c + 1][i], ring[s1][i], d1);
+        }
+    }
+    const float approx = fmaxf(fma_(-2.0f, d0 + d1, N + nr), 0.0f);
+    N -= csq[U & 7];  // column leaving the window of the next candidate
+    const bool ok = row_ok && col >= 0 && col <= W - 8;
+    if constexpr (kCount) cnt += ok ? 1u : 0u;
+    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | idx;
+    kl.push(ok ? key : 0xFFFFFFFFu);
+}
+
+// Approximate scan of the window of radius `range` around (cx, cy) into kl;
+// scan index of (dyi, dxi) is idx_base + dyi * n + dxi.
+template <int K, bool kCount>
+__device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, const PairGeom& g,
+                                          const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
+                                          int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
+    const int n = 2 * range + 1;
+    f16x2 ring[8][4];
+    float csq[8];
+    for (int dyi = 0; dyi < n; ++dyi) {
+        const int row = cy - range + dyi;
+        const bool row_ok = row >= 0 && row <= H - 8;
+        const int rt = row - g.ty0;
+        const int off = (rt & 1) * g.plane + (rt >> 1);
+        const int tc0 = cx - range - g.tx0;
+        const int q = tc0 / g.bs;
+        int ph = tc0 - q * g.bs;
+        int colbase = (ph * g.wq + q) * g.hp2;
+        float N = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            load_pair_col(ring[s], lds + off + colbase);
+            next_pair_col(g, ph, colbase);
+            csq[s] = col_sq(ring[s]);
+            N += csq[s];
+        }
+        const int col0 = cx - range;
+        const uint32_t rowidx = idx_base + (uint32_t)(dyi * n);
+        for (int base = 0; base < n; base += 8) {
+#define BM3D_DX_STEP(U)                                                                                 \
+    if (base + U < n)                                                                                   \
+        dx_step<U, K, kCount>(lds, g, rp, nr, ring, csq, N, ph, colbase, off, kl, col0 + base + U, row_ok, \
+                              W, rowidx + (uint32_t)(base + U), cnt);
+            BM3D_DX_STEP(0)
+            BM3D_DX_STEP(1)
+            BM3D_DX_STEP(2)
+            BM3D_DX_STEP(3)
+            BM3D_DX_STEP(4)
+            BM3D_DX_STEP(5)
+            BM3D_DX_STEP(6)
+            BM3D_DX_STEP(7)
+#undef BM3D_DX_STEP
+        }
+    }
+}
+
+// Exact distance between the block at (x, y) of `rplane` and the candidate
+// at (cx, cy) of `plane`, in the order of cpu_source/source.cpp:123-136 +
+// 98-103 (as ring_ssd); rows read from memory as two 16-byte loads each.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ float exact_ssd(const float* __restrict__ rplane, int x, int y,
+                                           const float* __restrict__ plane, int stride, int cx, int cy) {
+    const float* p = plane + (size_t)cy * stride + cx;
+    const float* q = rplane + (size_t)y * stride + x;
+    float d[8][8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const f4u p0 = *reinterpret_cast<const f4u*>(p + j * stride);
+        const f4u p1 = *reinterpret_cast<const f4u*>(p + j * stride + 4);
+        const f4u q0 = *reinterpret_cast<const f4u*>(q + j * stride);
+        const f4u q1 = *reinterpret_cast<const f4u*>(q + j * stride + 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            d[j][c] = q0[c] - p0[c];
+            d[j][c + 4] = q1[c] - p1[c];
+        }
+    }
+    float s[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float e0 = d[0][c] * d[0][c];
+        e0 = fma_(d[2][c], d[2][c], e0);
+        e0 = fma_(d[4][c], d[4][c], e0);
+        e0 = fma_(d[6][c], d[6][c], e0);
+        float e1 = d[1][c] * d[1][c];
+        e1 = fma_(d[3][c], d[3][c], e1);
+        e1 = fma_(d[5][c], d[5][c], e1);
+        e1 = fma_(d[7][c], d[7][c], e1);
+        s[c] = e0 + e1;
+    }
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+// Exhaustive exact scan of the window of radius `range` around (cx, cy) in
+// scan order (the FP32 kernel's scan_window, reading memory directly): the
+// path of slots whose range is too wide for FP16.
+template <bool kCount>
+__device__ void exact_window(const float* __restrict__ rplane, int x, int y, const float* __restrict__ plane,
+                             int stride, int cx, int cy, int range, int z, int W, int H, Best8& best,
+                             unsigned& cnt) {
+    for (int dy = -range; dy <= range; ++dy) {
+        const int Y = cy + dy;
+        if (Y < 0 || Y > H - 8) continue;
+        for (int dx = -range; dx <= range; ++dx) {
+            const int X = cx + dx;
+            if (X < 0 || X > W - 8) continue;
+            if constexpr (kCount) ++cnt;
+            const float e = exact_ssd(rplane, x, y, plane, stride, X, Y);
+            if (e < best.e[7]) best.insert(e, pack_pos(X, Y, z));
+        }
+    }
+}
+
+// r' = r - mu as packed vertical pairs: rp[c][i] = (r'[2i][c], r'[2i+1][c]).
+__device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __restrict__ plane, int stride, int x,
+                                           int y, float mu) {
+    float nr = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const _Float16 a = (_Float16)(plane[(size_t)(y + 2 * i) * stride + x + c] - mu);
+            const _Float16 b = (_Float16)(plane[(size_t)(y + 2 * i + 1) * stride + x + c] - mu);
+            rp[c][i] = f16x2{a, b};
+            nr = dot2(rp[c][i], rp[c][i], nr);
+        }
+    return nr;
+}
+
+}  // namespace
+
+// One lane per reference block; grid (ceil(nbx / 16), ceil(nby / TBY),
+// frames). kKt: keys kept per temporal frame (>= 2 ps_num).
+template <bool kCount, int kKt>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx(MatchArgs a) {
+    extern __shared__ __align__(16) float lds_f[];
+    uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
+    __shared__ uint32_t centre_s[8][256];  // the spatial list, per lane
+    const int tby = blockDim.x >> 4;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int f = blockIdx.z;
+    const int n = a.first_frame + f;
+    const int r = a.radius;
+    const int bx0 = blockIdx.x * 16, by0 = blockIdx.y * tby;
+    const int bxi = bx0 + lx, byi = by0 + ly;
+    const bool active = bxi < a.nbx && byi < a.nby;
+    const int x = min(min(bxi, a.nbx - 1) * a.bs, a.W - 8);
+    const int y = min(min(byi, a.nby - 1) * a.bs, a.H - 8);
+
+    TileGeom g;
+    g.tx0 = min(bx0 * a.bs, a.W - 8) - a.margin;
+    g.ty0 = min(by0 * a.bs, a.H - 8) - a.margin;
+    g.tw = a.tile_w;
+    g.th = a.tile_h;
+    g.bs = a.bs;
+    g.wq = a.tile_wq;
+    g.hp = a.tile_hp;
+    PairGeom pg;
+    pg.tx0 = g.tx0;
+    pg.ty0 = g.ty0;
+    pg.tw = g.tw;
+    pg.th = g.th;
+    pg.bs = g.bs;
+    pg.wq = g.wq;
+    pg.hp2 = a.tile_hp2;
+    pg.plane = a.bs * a.tile_wq * a.tile_hp2;
+
+    auto slot_plane = [&](int z) -> const float* {
+        const int fi = clampi(n - r + z, 0, a.num_frames - 1);
+        return a.clip + (size_t)fi * a.frame_stride;
+    };
+    const float* plane_r = slot_plane(r);
+
+    // Stages slot z as FP16 pairs offset by the tile's centre pixel mu;
+    // returns false (the slot is then scanned exactly from memory) when a
+    // pixel is not finite or further than kDxMaxSpan from mu.
+    auto stage = [&](const float* img, float& mu) -> bool {
+        const int cxm = clampi(pg.tx0 + pg.tw / 2, 0, a.W - 1), cym = clampi(pg.ty0 + pg.th / 2, 0, a.H - 1);
+        mu = img[(size_t)cym * a.stride + cxm];
+        if (!(__builtin_fabsf(mu) <= 1e30f)) mu = 0.0f;
+        return stage_pairs(lds, pg, img, a.W, a.H, a.stride, mu);
+    };
+
+    unsigned cnt = 0;
+    Best8 best;
+    {
+        float mu;
+        const bool fp16 = stage(plane_r, mu);
+        if (fp16) {
+            f16x2 rp[8][4];
+            const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
+            KeyList<kKSpatial> kl;
+            kl.init();
+            dx_window<kKSpatial, kCount>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
+            Best8Id bi;
+            bi.init(kFltMax, pack_pos(x, y, r));
+            const int nW = 2 * a.bm_range + 1;
+            const float inv_nW = 1.0f / (float)nW;
+#pragma unroll 1
+            for (int i = 0; i < kKSpatial; ++i) {
+                const uint32_t key = kl.pop();
+                if (key != 0xFFFFFFFFu) {
+                    const uint32_t idx = key & kIdxMask;
+                    const int dyi = (int)(((float)idx + 0.5f) * inv_nW), dxi = (int)idx - dyi * nW;
+                    const int cx = x - a.bm_range + dxi, cy = y - a.bm_range + dyi;
+                    const float e = exact_ssd(plane_r, x, y, plane_r, a.stride, cx, cy);
+                    if (e < bi.b.e[7] || (e == bi.b.e[7] && idx < bi.id[7])) bi.insert(e, pack_pos(cx, cy, r), idx);
+                }
+            }
+            best = bi.b;
+        } else {
+            best.init(kFltMax, pack_pos(x, y, r));
+            exact_window<kCount>(plane_r, x, y, plane_r, a.stride, x, y, a.bm_range, r, a.W, a.H, best, cnt);
+        }
+    }
+
+    if (r > 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) centre_s[i][threadIdx.x] = best.p[i];
+        const int nWt = 2 * a.ps_range + 1;
+        for (int dir = -1; dir <= 1; dir += 2) {
+            uint32_t last[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) last[i] = centre_s[i][threadIdx.x];
+            for (int t = 1; t <= r; ++t) {
+                const int z = r + dir * t;
+                const float* plane_z = slot_plane(z);
+                __syncthreads();
+                float mu;
+                const bool fp16 = stage(plane_z, mu);
+                Best8 fr;
+                fr.init(kFltMax, pack_pos(0, 0, z));
+                if (fp16) {
+                    f16x2 rp[8][4];
+                    const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
+                    KeyList<kKt> kl;
+                    kl.init();
+                    for (int i = 0; i < a.ps_num; ++i) {
+                        uint32_t p = last[0];
+#pragma unroll
+                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                        dx_window<kKt, kCount>(lds, pg, rp, nr, kl, pos_x(p), pos_y(p), a.ps_range, a.W, a.H,
+                                               (uint32_t)(i * nWt * nWt), cnt);
+                    }
+                    Best8Id fi;
+                    fi.init(kFltMax, pack_pos(0, 0, z));
+                    const float inv_nWt = 1.0f / (float)nWt, inv_nWt2 = 1.0f / (float)(nWt * nWt);
+#pragma unroll 1
+                    for (int i = 0; i < kKt; ++i) {
+                        const uint32_t key = kl.pop();
+                        if (key != 0xFFFFFFFFu) {
+                            const int idx = (int)(key & kIdxMask);
+                            const int w = (int)(((float)idx + 0.5f) * inv_nWt2);
+                            const int e2 = idx - w * nWt * nWt;
+                            const int dyi = (int)(((float)e2 + 0.5f) * inv_nWt), dxi = e2 - dyi * nWt;
+                            uint32_t p = last[0];
+#pragma unroll
+                            for (int k = 1; k < 8; ++k) p = (w == k) ? last[k] : p;
+                            const int cx = pos_x(p) - a.ps_range + dxi, cy = pos_y(p) - a.ps_range + dyi;
+                            const float e = exact_ssd(plane_r, x, y, plane_z, a.stride, cx, cy);
+                            if (e < fi.b.e[7] || (e == fi.b.e[7] && (uint32_t)idx < fi.id[7]))
+                                fi.insert(e, pack_pos(cx, cy, z), (uint32_t)idx);
+                        }
+                    }
+                    fr = fi.b;
+                } else {
+                    for (int i = 0; i < a.ps_num; ++i) {
+                        uint32_t p = last[0];
+#pragma unroll
+                        for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
+                        exact_window<kCount>(plane_r, x, y, plane_z, a.stride, pos_x(p), pos_y(p), a.ps_range, z, a.W,
+                                             a.H, fr, cnt);
+                    }
+                }
+                for (int i = 0; i < a.ps_num; ++i) {
+                    const float e = fr.err_at(i);
+                    if (e < best.e[7]) best.insert(e, fr.pos_at(i));
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) last[i] = fr.p[i];
+            }
+        }
+    }
+
+    // centre insert (source/kernel.cu:616-647)
+    const uint32_t cpos = pack_pos(x, y, r);
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) found |= best.p[i] == cpos;
+    if (!found) {
+#pragma unroll
+        for (int i = 7; i >= 1; --i) best.p[i] = best.p[i - 1];
+        best.p[0] = cpos;
+    }
+    if constexpr (kCount) {
+        if (active) atomicAdd(a.ncand, (unsigned long long)cnt);
+    }
+    if (active && a.groups) {
+        uint4* out = reinterpret_cast<uint4*>(a.groups + ((size_t)f * a.nblocks + (size_t)byi * a.nbx + bxi) * 8);
+        out[0] = make_uint4(best.p[0], best.p[1], best.p[2], best.p[3]);
+        out[1] = make_uint4(best.p[4], best.p[5], best.p[6], best.p[7]);
+    }
+}
+
+namespace {
+template <int kKt>
+hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
+    dim3 block(16 * tby);
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt>), grid, block, lds_bytes, s, a);
+    return hipGetLastError();
+}
+template <int kKt>
+hipError_t set_dx_limit(size_t bytes) {
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt>)};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+}  // namespace
+
+int dx_temporal_keys(int ps_num) { return ps_num <= 2 ? 4 : (ps_num <= 4 ? 8 : 16); }
+
+hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    switch (dx_temporal_keys(a.ps_num)) {
+        case 4: return launch_dx<4>(a, tby, lds_bytes, frames, s);
+        case 8: return launch_dx<8>(a, tby, lds_bytes, frames, s);
+        default: return launch_dx<16>(a, tby, lds_bytes, frames, s);
+    }
+}
+
+hipError_t set_match_dx_lds_limit(size_t bytes) {
+    hipError_t e = set_dx_limit<4>(bytes);
+    if (e == hipSuccess) e = set_dx_limit<8>(bytes);
+    if (e == hipSuccess) e = set_dx_limit<16>(bytes);
+    return e;
+}
+
+}  // namespace bm3d
