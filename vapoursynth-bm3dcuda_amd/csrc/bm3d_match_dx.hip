@@ -1,9 +1,9 @@
 // Mixed-precision block matching (BM3D_PRECISION_MIXED).
 //
 // The search of source/kernel.cu:432-647 (cpu_source/source.cpp:143-321)
-// with the candidate distances pre-scored in FP16 and only the best 2k
-// rescored exactly in the reference's FP32 order (k = 8 for the spatial
-// search, k = ps_num for each temporal frame: only the first ps_num entries
+// with the candidate distances pre-scored in FP16 and only the best few
+// rescored exactly in the reference's FP32 order (12 for the 8 of the spatial
+// search, 2 ps_num for the ps_num kept of each temporal frame: only the first ps_num entries
 // of a frame's list are merged into the group and centre the next frame's
 // windows, source/kernel.cu:568-611).
 //
@@ -44,7 +44,7 @@ constexpr float kInf = __builtin_huge_valf();
 constexpr float kDxMaxSpan = 8.0f;
 constexpr uint32_t kIdxBits = 12;
 constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1;
-constexpr int kKSpatial = 16;  // 2 x 8
+constexpr int kKSpatial = 12;  // keys kept in the spatial search (>= 8)
 
 __device__ __forceinline__ float dot2(f16x2 a, f16x2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
 __device__ __forceinline__ f16x2 as_h2(uint32_t u) { return __builtin_bit_cast(f16x2, u); }
@@ -156,14 +156,15 @@ __device__ __forceinline__ void load_pair_col(f16x2 (&dst)[4], const uint32_t* _
     for (int i = 0; i < 4; ++i) dst[i] = as_h2(src[i]);
 }
 
+// One candidate step: loads the window's new column (word address `addr`),
+// updates the column sums and pushes the candidate's key. Step u of the
+// row is an in-plane candidate iff u - vlo <= vspan (unsigned).
 template <int U, int K, bool kCount>
-__device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const PairGeom& g,
-                                        const f16x2 (&rp)[8][4], float nr, f16x2 (&ring)[8][4],
-                                        float (&csq)[8], float& N, int& ph, int& colbase, int off,
-                                        KeyList<K>& kl, int col, bool row_ok, int W, uint32_t idx,
+__device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const f16x2 (&rp)[8][4], float nr,
+                                        f16x2 (&ring)[8][4], float (&csq)[8], float& N, int addr,
+                                        KeyList<K>& kl, int u, unsigned vlo, unsigned vspan, uint32_t idx,
                                         unsigned& cnt) {
-    load_pair_col(ring[(U + 7) & 7], lds + off + colbase);
-    next_pair_col(g, ph, colbase);
+    load_pair_col(ring[(U + 7) & 7], lds + addr);
     csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
     N += csq[(U + 7) & 7];
     float d0 = 0.0f, d1 = 0.0f;
@@ -178,9 +179,9 @@ __device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const 
     }
     const float approx = fmaxf(fma_(-2.0f, d0 + d1, N + nr), 0.0f);
     N -= csq[U & 7];  // column leaving the window of the next candidate
-    const bool ok = row_ok && col >= 0 && col <= W - 8;
+    const bool ok = (unsigned)u - vlo <= vspan;
     if constexpr (kCount) cnt += ok ? 1u : 0u;
-    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | idx;
+    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | (idx & kIdxMask);
     kl.push(ok ? key : 0xFFFFFFFFu);
 }
 
@@ -191,11 +192,14 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
                                           const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
                                           int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
     const int n = 2 * range + 1;
+    // with block_step dividing 8, 8 columns on are the same phase 8 / bs
+    // slots further: the step addresses of a row advance by a constant
+    const bool div8 = (8 % g.bs) == 0;
+    const int inc8 = div8 ? (8 / g.bs) * g.hp2 : 0;
     f16x2 ring[8][4];
     float csq[8];
     for (int dyi = 0; dyi < n; ++dyi) {
         const int row = cy - range + dyi;
-        const bool row_ok = row >= 0 && row <= H - 8;
         const int rt = row - g.ty0;
         const int off = (rt & 1) * g.plane + (rt >> 1);
         const int tc0 = cx - range - g.tx0;
@@ -210,13 +214,26 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
             csq[s] = col_sq(ring[s]);
             N += csq[s];
         }
+        int ad[8];  // word addresses of the new column of steps base .. base + 7
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ad[k] = off + colbase;
+            next_pair_col(g, ph, colbase);
+        }
+        // in-plane steps: row inside, column col0 + u in [0, W - 8]
         const int col0 = cx - range;
+        const int lo = max(0, -col0), hi = min(n - 1, W - 8 - col0);
+        const bool any = row >= 0 && row <= H - 8 && lo <= hi;
+        const unsigned vlo = any ? (unsigned)lo : 0x40000000u;
+        const unsigned vspan = any ? (unsigned)(hi - lo) : 0u;
         const uint32_t rowidx = idx_base + (uint32_t)(dyi * n);
-        for (int base = 0; base < n; base += 8) {
-#define BM3D_DX_STEP(U)                                                                                 \
-    if (base + U < n)                                                                                   \
-        dx_step<U, K, kCount>(lds, g, rp, nr, ring, csq, N, ph, colbase, off, kl, col0 + base + U, row_ok, \
-                              W, rowidx + (uint32_t)(base + U), cnt);
+        // whole groups of 8 steps unconditionally (the ring returns to the
+        // same registers after 8), then the 0..7 remaining steps
+        int base = 0;
+#define BM3D_DX_STEP(U)                                                                              \
+    dx_step<U, K, kCount>(lds, rp, nr, ring, csq, N, ad[U], kl, base + U, vlo, vspan,              \
+                          rowidx + (uint32_t)(base + U), cnt);
+        for (; base + 8 <= n; base += 8) {
             BM3D_DX_STEP(0)
             BM3D_DX_STEP(1)
             BM3D_DX_STEP(2)
@@ -225,8 +242,30 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
             BM3D_DX_STEP(5)
             BM3D_DX_STEP(6)
             BM3D_DX_STEP(7)
-#undef BM3D_DX_STEP
+            if (div8) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) ad[k] += inc8;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    ad[k] = off + colbase;
+                    next_pair_col(g, ph, colbase);
+                }
+            }
         }
+        switch (n - base) {
+            case 7: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4)
+                    BM3D_DX_STEP(5) BM3D_DX_STEP(6) break;
+            case 6: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4)
+                    BM3D_DX_STEP(5) break;
+            case 5: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4) break;
+            case 4: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) break;
+            case 3: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) break;
+            case 2: BM3D_DX_STEP(0) BM3D_DX_STEP(1) break;
+            case 1: BM3D_DX_STEP(0) break;
+            default: break;
+        }
+#undef BM3D_DX_STEP
     }
 }
 
