@@ -5,8 +5,11 @@ Workload (one step): GRAYS float32 1920x1080, 30-frame synthetic video
 (frame 0 = sinusoids + rectangles, later frames shifted by integer motion
 U{-3..3}, fresh N(0, 15/255) noise per frame, seed 1), parameters sigma=15,
 block_step=4, bm_range=9, ps_num=2, ps_range=4, radius=1:
-    basic = BM3Dv2(src, radius=1)            -> BM3D stacks + VAggregate
-    final = BM3Dv2(src, ref=basic, radius=1) -> BM3D stacks + VAggregate
+    basic = BM3Dv2(src, radius=1)            -> bm3d_execute_vagg
+    final = BM3Dv2(src, ref=basic, radius=1) -> bm3d_execute_vagg
+(each pass runs the BM3D evaluations over a ring of frames and the
+VAggregate average straight from their accumulation regions; the per-frame
+stacks of the two-stage form are never written -- the outputs are identical).
 The clip is resident in HBM before the timed region; value = frames/s of
 the whole two-pass job over K steps, for the mixed-precision path
 (--precision, default mixed: FP16 block-matching pre-scoring with exact FP32
@@ -113,17 +116,14 @@ class Pipeline:
                             p["ps_num"], p["ps_range"], final=False, precision=precision)
         self.pf = core.Plan(w, h, p["sigma"], p["block_step"], p["bm_range"], p["radius"],
                             p["ps_num"], p["ps_range"], final=True, precision=precision)
-        self.stacks = self.pb.new_stacks(n)
         import torch
         self.basic = torch.empty_like(src)
         self.final = torch.empty_like(src)
 
     def step(self, stream=None):
-        r = PARAMS["radius"]
-        self.pb.execute(self.src, out=self.stacks, stream=stream)
-        self.core.vaggregate(self.stacks, r, self.n, out=self.basic, stream=stream)
-        self.pf.execute(self.src, self.basic, out=self.stacks, stream=stream)
-        self.core.vaggregate(self.stacks, r, self.n, out=self.final, stream=stream)
+        # BM3Dv2 = BM3D + VAggregate, fused: the stacks are not materialised
+        self.pb.execute_vagg(self.src, out=self.basic, stream=stream)
+        self.pf.execute_vagg(self.src, self.basic, out=self.final, stream=stream)
 
 
 def time_path(core, capi, src, precision, steps, warmup):

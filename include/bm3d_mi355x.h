@@ -104,6 +104,21 @@ long long bm3d_stack_size(const bm3d_plan *plan);
 int bm3d_execute(bm3d_plan *plan, float *d_res, long long res_frame_stride, long long res_plane_stride,
                  const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count, void *stream);
 
+/* BM3D followed by VAggregate (the BM3Dv2 composition, source/source.cpp:
+ * 944-988) for output frames first_frame .. first_frame+count-1, radius > 0:
+ * dst frame i, plane p at dst + i * dst_frame_stride + p * dst_plane_stride,
+ * rows dst_stride floats apart (0 = packed, width floats). The same values
+ * as bm3d_execute into stacks followed by bm3d_vaggregate (bit for bit when
+ * extractor_exp is set), computed without writing the stacks: each output
+ * reads the accumulation of the 2r + 1 evaluations that cover it. Planes
+ * skipped by chroma (sigma < FLT_EPSILON) are not written. Returns
+ * BM3D_EUNSUPPORTED when the plan's accumulation tiles cannot hold every
+ * group member (very large bm_range + radius * ps_range); use bm3d_execute
+ * and bm3d_vaggregate then. No counterpart entry point in the reference. */
+int bm3d_execute_vagg(bm3d_plan *plan, float *dst, long long dst_frame_stride, long long dst_plane_stride,
+                      int dst_stride, const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count,
+                      void *stream);
+
 /* One window in the reference's packed input layout
  * d_src = [(final ? 2 : 1)][planes][2r+1][height][stride], ref first
  * (source/kernel.cu:43-44). Writes one packed stack to d_res. */

@@ -160,6 +160,37 @@ def test_deterministic_with_extractor():
     assert synth.sha256(a) == synth.sha256(b)
 
 
+# ------------------------------------------------ BM3D + VAggregate fused
+@pytest.mark.parametrize("extractor_exp", [10, 0])
+@pytest.mark.parametrize("case", [
+    (48, 64, 5, 1, False, dict(sigma=[12.0], block_step=4, bm_range=5, radius=1, ps_num=2, ps_range=3)),
+    (40, 56, 6, 1, False, dict(sigma=[15.0], block_step=3, bm_range=4, radius=2, ps_num=2, ps_range=2)),
+    (33, 47, 3, 1, False, dict(sigma=[20.0], block_step=5, bm_range=3, radius=2, ps_num=3, ps_range=2)),
+    (40, 48, 4, 3, True, dict(sigma=[10.0, 0.0, 12.0], block_step=4, bm_range=4, radius=1, ps_num=2,
+                              ps_range=2)),
+], ids=["r1", "r2", "r2-3frames", "chroma-r1"])
+def test_execute_vagg_equals_stacks_then_vaggregate(case, extractor_exp):
+    """bm3d_execute_vagg gives the values of bm3d_execute + bm3d_vaggregate,
+    for the whole clip and for frame subranges (evaluations of neighbouring
+    frames are recomputed at the range edges)."""
+    core = _gpu()
+    H, W, N, P, chroma, prm = case
+    _, noisy = synth.make_clip(H, W, N, P, noise_sd=prm["sigma"][0], seed=5, motion=True, yuv=chroma)
+    src = torch.from_numpy(noisy).cuda()
+    plan = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
+                     prm["ps_num"], prm["ps_range"], chroma=chroma, extractor_exp=extractor_exp)
+    want = core.vaggregate(plan.execute(src), prm["radius"], N).cpu().numpy()
+    keep = [p for p in range(P) if not (chroma and prm["sigma"][p] == 0.0)]
+    for first, count in [(0, N), (1, N - 1), (N - 1, 1), (N // 2, 1)]:
+        got = plan.execute_vagg(src, first=first, count=count).cpu().numpy()
+        w = want[first:first + count][:, keep]
+        g = got[:, keep]
+        if extractor_exp:
+            np.testing.assert_array_equal(g, w)
+        else:
+            np.testing.assert_allclose(g, w, rtol=2e-6, atol=5e-6)
+
+
 # ---------------------------------------------------------- mixed precision
 # The mixed path pre-scores candidates in FP16 and rescores the best 2k
 # exactly (k = 8 spatial, k = ps_num temporal); its lists equal the FP32

@@ -100,6 +100,9 @@ def lib():
     L.bm3d_execute.argtypes = [P, P, c_ll, c_ll, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
                                c_int, c_int, P]
     L.bm3d_execute.restype = c_int
+    L.bm3d_execute_vagg.argtypes = [P, P, c_ll, c_ll, c_int, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
+                                    c_int, c_int, P]
+    L.bm3d_execute_vagg.restype = c_int
     L.bm3d_execute_window.argtypes = [P, P, P, P]
     L.bm3d_execute_window.restype = c_int
     L.bm3d_match.argtypes = [P, P, ctypes.POINTER(Clip), ctypes.POINTER(Clip), c_int, c_int, P]
@@ -151,7 +154,7 @@ def check(rc):
 
 # Exported names of include/bm3d_mi355x.h (checked by the CPU test suite).
 EXPORTS = (
-    "bm3d_plan_create", "bm3d_plan_destroy", "bm3d_stack_size", "bm3d_execute",
+    "bm3d_plan_create", "bm3d_plan_destroy", "bm3d_stack_size", "bm3d_execute", "bm3d_execute_vagg",
     "bm3d_execute_window", "bm3d_aggregate", "bm3d_vaggregate", "bm3d_match",
     "bm3d_num_blocks", "bm3d_plan_matching_kernel", "bm3d_last_error", "bm3d_version", "bm3d_profile_enable",
     "bm3d_profile_reset", "bm3d_profile_read", "bm3d_count_candidates",

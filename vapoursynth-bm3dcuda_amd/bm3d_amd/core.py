@@ -99,6 +99,24 @@ class Plan:
                                      int(first), int(count), _stream_ptr(stream)))
         return out
 
+    def execute_vagg(self, src, ref=None, first=0, count=None, out=None, stream=None):
+        """BM3D followed by VAggregate (radius > 0) for frames first ..
+        first + count - 1, without materialising the stacks:
+        [count, planes, H, W] float32."""
+        _check_clip(src, "clip")
+        n = src.shape[0]
+        count = n - first if count is None else count
+        d = self.desc
+        if out is None:
+            out = torch.empty((count, self.nplanes, d.height, d.width), dtype=torch.float32, device=src.device)
+        s = clip_desc(src)
+        r = clip_desc(ref) if ref is not None else None
+        C.check(C.lib().bm3d_execute_vagg(self._h, ctypes.c_void_p(out.data_ptr()), out.stride(0),
+                                          out.stride(1), out.stride(2), ctypes.byref(s),
+                                          ctypes.byref(r) if r else None, int(first), int(count),
+                                          _stream_ptr(stream)))
+        return out
+
     def uses_mixed_kernel(self):
         """True when block matching runs on the mixed-precision kernel."""
         return bool(C.lib().bm3d_plan_matching_kernel(self._h))

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -219,6 +220,66 @@ hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long l
     return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
 }
 
+// Block matching and filtering of frames n0 .. n0 + nf - 1: group lists,
+// the tile accumulation regions (scratch) and the corner weight maps. With
+// ring > 0, frame n's regions and corner maps go to index n % ring of the
+// plan's buffers (which hold `ring` frames), else to index n - n0.
+hipError_t evaluate(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int n0, int nf, float* res,
+                    long long res_frame_stride, long long res_plane_stride, int ring, hipStream_t stream) {
+    const bm3d_desc& d = p->d;
+    hipError_t e = hipSuccess;
+    if (ring > 0) {
+        // corner maps of the ring slots of n0 .. n0 + nf - 1 (may wrap)
+        int f = 0;
+        while (f < nf && e == hipSuccess) {
+            const int slot = (n0 + f) % ring;
+            const int run = std::min(nf - f, ring - slot);
+            e = bm3d::launch_zero(p->corner + (size_t)slot * p->corner_frame, (size_t)run * p->corner_frame, 0, 1,
+                                  stream);
+            f += run;
+        }
+    } else {
+        e = bm3d::launch_zero(p->corner, (size_t)nf * p->corner_frame, 0, 1, stream);
+    }
+    if (e != hipSuccess) return e;
+    {
+        ProfScope ps(0, stream);
+        e = launch_matching(p, p->groups, nullptr, src, ref, n0, nf, stream);
+    }
+    if (e != hipSuccess) return e;
+    bm3d::FilterArgs fa{};
+    fa.src = src->data;
+    fa.ref = d.final_ ? ref->data : nullptr;
+    fa.frame_stride = src->frame_stride;
+    fa.plane_stride = src->plane_stride;
+    fa.num_frames = src->num_frames;
+    fa.first_frame = n0;
+    fa.W = d.width; fa.H = d.height; fa.stride = d.stride;
+    fa.plane_size = p->plane_size;
+    fa.radius = d.radius;
+    fa.nblocks = p->nblocks;
+    for (int i = 0; i < 3; ++i) fa.sigma[i] = p->sigma[i];
+    fa.extractor = p->extractor;
+    fa.groups = p->groups;
+    fa.res = res;
+    fa.res_frame_stride = res_frame_stride;
+    fa.res_plane_stride = res_plane_stride;
+    fa.corner = p->corner;
+    fa.corner_frame_stride = p->corner_frame;
+    fa.bs = d.block_step; fa.nbx = p->nbx; fa.nby = p->nby;
+    fa.tbx = p->ftbx; fa.tby = p->ftby; fa.ntx = p->fntx; fa.nty = p->fnty;
+    fa.rm = p->frm; fa.rw = p->frw; fa.rh = p->frh;
+    fa.scratch = p->scratch;
+    fa.fx_scale = 1.0f / p->fx_unit;
+    fa.ring = ring;
+    {
+        ProfScope ps(1, stream);
+        e = bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream);
+    }
+    return e;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -375,41 +436,7 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
                                                (size_t)res_plane_stride, p->nplanes, stream));
             }
         }
-        BM3D_HIP(bm3d::launch_zero(p->corner, (size_t)nf * p->corner_frame, 0, 1, stream));
-
-        {
-            ProfScope ps(0, stream);
-            BM3D_HIP(launch_matching(p, p->groups, nullptr, src, ref, n0, nf, stream));
-        }
-
-        bm3d::FilterArgs fa{};
-        fa.src = src->data;
-        fa.ref = d.final_ ? ref->data : nullptr;
-        fa.frame_stride = src->frame_stride;
-        fa.plane_stride = src->plane_stride;
-        fa.num_frames = src->num_frames;
-        fa.first_frame = n0;
-        fa.W = d.width; fa.H = d.height; fa.stride = d.stride;
-        fa.plane_size = p->plane_size;
-        fa.radius = d.radius;
-        fa.nblocks = p->nblocks;
-        for (int i = 0; i < 3; ++i) fa.sigma[i] = p->sigma[i];
-        fa.extractor = p->extractor;
-        fa.groups = p->groups;
-        fa.res = res;
-        fa.res_frame_stride = res_frame_stride;
-        fa.res_plane_stride = res_plane_stride;
-        fa.corner = p->corner;
-        fa.corner_frame_stride = p->corner_frame;
-        fa.bs = d.block_step; fa.nbx = p->nbx; fa.nby = p->nby;
-        fa.tbx = p->ftbx; fa.tby = p->ftby; fa.ntx = p->fntx; fa.nty = p->fnty;
-        fa.rm = p->frm; fa.rw = p->frw; fa.rh = p->frh;
-        fa.scratch = p->scratch;
-        fa.fx_scale = 1.0f / p->fx_unit;
-        {
-            ProfScope ps(1, stream);
-            BM3D_HIP(bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream));
-        }
+        BM3D_HIP(evaluate(p, src, ref, n0, nf, res, res_frame_stride, res_plane_stride, 0, stream));
 
         bm3d::FinalizeArgs fz{};
         fz.scratch = p->scratch;
@@ -515,6 +542,81 @@ int bm3d_match(bm3d_plan* p, unsigned int* d_groups, const bm3d_clip* src, const
     if (count == 0) return BM3D_OK;
     BM3D_HIP(launch_matching(p, d_groups, nullptr, src, ref, first_frame, count,
                              reinterpret_cast<hipStream_t>(stream)));
+    return BM3D_OK;
+}
+
+int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long long dst_plane_stride,
+                      int dst_stride, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
+                      void* stream_) {
+    g_last_error.clear();
+    if (!p || !dst || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
+    const bm3d_desc& d = p->d;
+    if (d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (d.final_ && (ref->frame_stride != src->frame_stride || ref->plane_stride != src->plane_stride ||
+                     ref->num_frames != src->num_frames))
+        return fail(BM3D_EINVAL, "BM3D: \"ref\" must have the layout and number of frames of \"clip\"");
+    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    if (d.radius <= 0) return fail(BM3D_EINVAL, "BM3D: bm3d_execute_vagg needs radius > 0");
+    if (p->frm < p->margin)
+        return fail(BM3D_EUNSUPPORTED, "BM3D: search reach too large for region aggregation "
+                                       "(use bm3d_execute and bm3d_vaggregate)");
+    if (dst_stride == 0) dst_stride = d.width;
+    if (dst_stride < d.width) return fail(BM3D_EINVAL, "BM3D: destination stride < width");
+    if (dst_plane_stride == 0) dst_plane_stride = (long long)d.height * dst_stride;
+    if (dst_frame_stride == 0) dst_frame_stride = p->nplanes * dst_plane_stride;
+    if (count == 0) return BM3D_OK;
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    int dev = 0;
+    BM3D_HIP(hipGetDevice(&dev));
+    if (dev != p->device) return fail(BM3D_EINVAL, "BM3D: plan used on a different device than it was created on");
+
+    const int N = src->num_frames, r = d.radius;
+    // evaluations needed: frames clamp(n - r .. n + r) of every output n
+    const int mlo = std::max(first_frame - r, 0);
+    const int mhi = std::min(first_frame + count - 1 + r, N - 1);
+    const int batch = std::min(kMaxBatchFrames, mhi - mlo + 1);
+    // resident evaluations: an output's 2r + 1 plus the batch in flight
+    const int ring = std::min(batch + 2 * r, mhi - mlo + 1);
+    if (int rc = ensure_scratch(p, ring)) return rc;
+
+    bm3d::VRegArgs va{};
+    va.scratch = p->scratch;
+    va.scratch_frame = p->scratch_frame;
+    va.corner = p->corner;
+    va.corner_frame = p->corner_frame;
+    va.ring = ring;
+    va.fx_unit = p->fx_unit;
+    va.dst_frame_stride = dst_frame_stride;
+    va.dst_plane_stride = dst_plane_stride;
+    va.dst_stride = dst_stride;
+    va.W = d.width; va.H = d.height; va.stride = d.stride;
+    va.plane_size = p->plane_size;
+    va.nplanes = p->nplanes;
+    va.plane_mask = 0;
+    for (int i = 0; i < p->nplanes; ++i)
+        if (!(d.chroma && p->sigma[i] < std::numeric_limits<float>::epsilon())) va.plane_mask |= 1u << i;
+    va.radius = r;
+    va.num_frames = N;
+    va.bs = d.block_step; va.tbx = p->ftbx; va.tby = p->ftby; va.ntx = p->fntx; va.nty = p->fnty;
+    va.rm = p->frm; va.rw = p->frw; va.rh = p->frh;
+
+    int next_out = first_frame;
+    for (int b0 = mlo; b0 <= mhi;) {
+        const int b1 = std::min(b0 + batch, mhi + 1);
+        BM3D_HIP(evaluate(p, src, ref, b0, b1 - b0, nullptr, 0, 0, ring, stream));
+        // outputs whose evaluations are all resident
+        const int out_hi = (b1 - 1 == N - 1) ? first_frame + count - 1
+                                              : std::min(first_frame + count - 1, b1 - 1 - r);
+        if (out_hi >= next_out) {
+            va.first_frame = next_out;
+            va.dst = dst + (size_t)(next_out - first_frame) * dst_frame_stride;
+            BM3D_HIP(bm3d::launch_vagg_regions(va, out_hi - next_out + 1, stream));
+            next_out = out_hi + 1;
+        }
+        b0 = b1;
+    }
+    if (next_out != first_frame + count) return fail(BM3D_EHIP, "BM3D: internal frame schedule error");
     return BM3D_OK;
 }
 

@@ -56,6 +56,7 @@ struct FilterArgs {
     int rm, rw, rh;          // accumulation region: margin, width, height
     int* scratch;            // [frames][nty][ntx][planes][2r+1][rh][rw], fixed point
     float fx_scale;          // float -> fixed-point units (1 / fx_unit)
+    int ring;                // > 0: frame n's scratch and corner maps at index n % ring
 };
 
 struct FinalizeArgs {
@@ -93,6 +94,26 @@ struct DivideArgs {
     int nplanes;
 };
 
+// VAggregate straight from the filter's accumulation regions and corner maps
+// (bm3d_vagg_regions): the stacks are never written.
+struct VRegArgs {
+    const int* scratch;      // ring of evaluations: [ring][nty][ntx][planes][2r+1][rh][rw]
+    long long scratch_frame; // ints per evaluation
+    const float* corner;     // [ring][planes][2r+1][H][stride]
+    long long corner_frame;  // floats per evaluation
+    int ring;                // evaluation of frame m at index m % ring
+    float fx_unit;
+    float* dst;              // output frames [first_frame, first_frame + count)
+    long long dst_frame_stride, dst_plane_stride;
+    int dst_stride;
+    int W, H, stride;
+    long long plane_size;
+    int nplanes;             // planes of the evaluation (1, or 3 with chroma)
+    unsigned plane_mask;     // planes to produce (bit p)
+    int radius, num_frames, first_frame;
+    int bs, tbx, tby, ntx, nty, rm, rw, rh;
+};
+
 struct VAggArgs {
     const float* stacks;     // stacks of frames [stack_first, ...)
     long long stack_frame_stride, stack_plane_stride;
@@ -123,5 +144,6 @@ hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s
 hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s);
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s);
 hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s);
+hipError_t launch_vagg_regions(const VRegArgs& a, int count, hipStream_t s);
 
 }  // namespace bm3d

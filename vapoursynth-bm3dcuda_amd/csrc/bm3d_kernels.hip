@@ -303,7 +303,8 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
         __syncthreads();
         const float* src = a.src + (size_t)p * a.plane_stride;
         float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
-        float* corner = a.corner + (size_t)f * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
+        const int slot = a.ring > 0 ? n % a.ring : f;
+        float* corner = a.corner + (size_t)slot * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
 
         for (int base = 0; base < ngroups; base += nwaves * 8) {
             const int li = base + wave * 8 + g;
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
                             val = (val + E) - E;
                             atomicAdd(q + j * a.rw, (int)__builtin_rintf(val * a.fx_scale));
                         }
-                    } else {
+                    } else if (a.res) {
                         float* q = res + (size_t)(2 * z) * a.plane_size + (size_t)py * a.stride + px + k;
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
             }
         }
         __syncthreads();
-        int* out = a.scratch + ((((size_t)f * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
+        int* out = a.scratch + ((((size_t)slot * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
         for (int i = threadIdx.x; i < region; i += blockDim.x) out[i] = acc[i];
         __syncthreads();
     }
@@ -566,6 +567,86 @@ __global__ void bm3d_vaggregate(VAggArgs a) {
     a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] = num / den;
 }
 
+// VAggregate (source/source.cpp:834-836) of the stacks bm3d_finalize and
+// bm3d_den_box would write, computed from their sources: for each slot i of
+// output frame n, the weighted sum is the integer sum of the covering tile
+// regions times fx_unit (as bm3d_finalize) and the weight is the 8x8 box sum
+// of the corner map (as bm3d_den_box, same order); both are added in slot
+// order and divided (as bm3d_vaggregate). Output tile 64 x 8, two pixels per
+// thread. Grid: (ceil(W / 64), ceil(H / 8), count * nplanes).
+__global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
+    __shared__ float rowsum[15][64];
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;  // 0..3
+    const int x = blockIdx.x * 64 + tx;
+    const int y0 = blockIdx.y * 8;
+    const int fo = blockIdx.z / a.nplanes, p = blockIdx.z % a.nplanes;
+    if (!((a.plane_mask >> p) & 1u)) return;  // uniform per work-group
+    const int n = a.first_frame + fo;
+    const int r = a.radius;
+    const int tw = 2 * r + 1;
+    const int region = tw * a.rh * a.rw;
+    const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
+    float num[2] = {0.0f, 0.0f}, den[2] = {0.0f, 0.0f};
+    for (int i = 0; i < tw; ++i) {
+        const int m = clampi(n - r + i, 0, a.num_frames - 1);
+        const int z = clampi(2 * r - i, n - a.num_frames + 1 + r, n + r);
+        const int slot = m % a.ring;
+        const float* cmap = a.corner + (size_t)slot * a.corner_frame + (size_t)(p * tw + z) * a.plane_size;
+        __syncthreads();
+        for (int rr = ty; rr < 15; rr += 4) {
+            const int yy = y0 - 7 + rr;
+            float s = 0.0f;
+            if (yy >= 0 && yy < a.H) {
+                const float* row = cmap + (size_t)yy * a.stride;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int xx = x - b;
+                    s = s + ((xx >= 0 && xx < a.W) ? row[xx] : 0.0f);
+                }
+            }
+            rowsum[rr][tx] = s;
+        }
+        __syncthreads();
+        const int* scr = a.scratch + (size_t)slot * a.scratch_frame;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int jj = ty + 4 * h;
+            const int y = y0 + jj;
+            if (y >= a.H || x >= a.W) continue;
+            float dv = 0.0f;
+#pragma unroll
+            for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
+            long long isum = 0;
+            const int ylo_n = y - a.rm - 7 - (a.tby - 1) * a.bs;
+            const int xlo_n = x - a.rm - 7 - (a.tbx - 1) * a.bs;
+            const int ty_lo = max((ylo_n < 0 ? 0 : ylo_n / th) - 1, 0);
+            const int ty_hi = min((y + a.rm) / th + 1, a.nty - 1);
+            const int tx_lo = max((xlo_n < 0 ? 0 : xlo_n / tw_) - 1, 0);
+            const int tx_hi = min((x + a.rm) / tw_ + 1, a.ntx - 1);
+            for (int tyy = ty_lo; tyy <= ty_hi; ++tyy) {
+                const int ly = y - (min(tyy * th, a.H - 8) - a.rm);
+                if (ly < 0 || ly >= a.rh) continue;
+                for (int txx = tx_lo; txx <= tx_hi; ++txx) {
+                    const int lx = x - (min(txx * tw_, a.W - 8) - a.rm);
+                    if (lx < 0 || lx >= a.rw) continue;
+                    const int* reg = scr + (((size_t)tyy * a.ntx + txx) * a.nplanes + p) * region;
+                    isum += reg[(z * a.rh + ly) * a.rw + lx];
+                }
+            }
+            num[h] = num[h] + (0.0f + (float)isum * a.fx_unit);
+            den[h] = den[h] + dv;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + ty + 4 * h;
+        if (y < a.H && x < a.W)
+            a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] =
+                num[h] / den[h];
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
@@ -635,6 +716,12 @@ hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s) {
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
     dim3 grid((a.W + 255) / 256, a.H, count);
     hipLaunchKernelGGL(bm3d_divide, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vagg_regions(const VRegArgs& a, int count, hipStream_t s) {
+    dim3 grid((a.W + 63) / 64, (a.H + 7) / 8, count * a.nplanes);
+    hipLaunchKernelGGL(bm3d_vagg_regions, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
