@@ -156,6 +156,19 @@ long long bm3d_num_blocks(const bm3d_plan *plan);
  * the mixed kernel's 12-bit scan indices). No counterpart in the reference. */
 int bm3d_plan_matching_kernel(const bm3d_plan *plan);
 
+/* Filtering kernel of the plan: 1 for the mixed-precision kernel (precision
+ * MIXED: the 3-D DCT pair as FP16 matrix products with FP32 accumulation;
+ * shrinkage, weights and aggregation in FP32), 0 for the FP32 kernel. No
+ * counterpart in the reference. */
+int bm3d_plan_filter_kernel(const bm3d_plan *plan);
+
+/* Selects the plan's kernels, for testing and measurement (no counterpart in
+ * the reference): `matching` and `filter` each 0 (FP32) or 1 (mixed
+ * precision). Mixed kernels need a plan created with precision MIXED
+ * (BM3D_EINVAL otherwise); the mixed matching kernel also needs search
+ * windows within its limits (BM3D_EUNSUPPORTED otherwise). */
+int bm3d_plan_select_kernels(bm3d_plan *plan, int matching, int filter);
+
 /* Kernel timing (no counterpart in the reference). While enabled, every
  * launch of the block-matching (kernel 0) and filtering (kernel 1) kernels
  * is bracketed by HIP events on its stream. bm3d_profile_read waits for the

@@ -31,7 +31,7 @@ def _window(clip, n, r):
     return np.stack([clip[min(max(n - r + z, 0), N - 1)] for z in range(2 * r + 1)], axis=1)
 
 
-def _oracle_stacks(noisy, ref, prm, final, chroma):
+def _oracle_stacks(noisy, ref, prm, final, chroma, dtype=np.float32):
     """Oracle stacks for every frame: [N, planes, 2r+1, 2, H, W]."""
     N = noisy.shape[0]
     r = prm["radius"]
@@ -42,7 +42,7 @@ def _oracle_stacks(noisy, ref, prm, final, chroma):
     for n in range(N):
         res, g, _ = O.bm3d_window(_window(noisy, n, r), _window(ref, n, r) if final else None, sig,
                                   prm["block_step"], prm["bm_range"], r, prm["ps_num"],
-                                  prm["ps_range"], chroma, final, E, want_groups=True)
+                                  prm["ps_range"], chroma, final, E, dtype=dtype, want_groups=True)
         out.append(res)
         groups.append(g)
     return np.stack(out), np.stack(groups)
@@ -131,6 +131,25 @@ def test_two_pass_stacks(case, extractor_exp):
         np.testing.assert_allclose(got_f, want_f, rtol=2e-6, atol=5e-6)
 
 
+@pytest.mark.parametrize("kind", ["x255", "offset1000"])
+def test_stacks_wide_range_match_oracle(kind):
+    """The fixed-point unit of the region accumulation follows the input's
+    range (largest finite |sample| of the call), so inputs far outside
+    [0, 1] -- which a fixed 2^-22 unit would overflow -- match the oracle's
+    float sums to the same relative tolerance."""
+    core = _gpu()
+    H, W, N = 40, 56, 3
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=17, motion=True)
+    scale, off = (255.0, 0.0) if kind == "x255" else (1.0, 1000.0)
+    x = (noisy * np.float32(scale) + np.float32(off)).astype(np.float32)
+    prm = dict(sigma=[15.0 * scale], block_step=4, bm_range=4, radius=1, ps_num=2, ps_range=3,
+               extractor_exp=0)
+    plan = _plan(core, W, H, prm, False, False)
+    got = plan.execute(torch.from_numpy(x).cuda()).cpu().numpy()
+    want, _ = _oracle_stacks(x, None, prm, False, False)
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=5e-6 * max(scale, off))
+
+
 def test_vaggregate_and_aggregate_match_oracle():
     core = _gpu()
     H, W, N = 24, 40, 5
@@ -147,17 +166,136 @@ def test_vaggregate_and_aggregate_match_oracle():
     np.testing.assert_array_equal(got0[:, 0], st0[:, 0, 0, 0] / st0[:, 0, 0, 1])
 
 
-def test_deterministic_with_extractor():
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_deterministic_with_extractor(precision):
     core = _gpu()
     H, W, N = 64, 96, 3
     prm = dict(sigma=[15.0], block_step=4, bm_range=6, radius=1, ps_num=2, ps_range=3,
                extractor_exp=8)
     _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=9, motion=True)
     src = torch.from_numpy(noisy).cuda()
-    plan = _plan(core, W, H, prm, False, False)
-    a = plan.execute(src).cpu().numpy()
-    b = plan.execute(src).cpu().numpy()
-    assert synth.sha256(a) == synth.sha256(b)
+    for final in (False, True):
+        plan = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
+                         prm["ps_num"], prm["ps_range"], final=final, extractor_exp=8,
+                         precision=precision)
+        ref = src if final else None
+        a = plan.execute(src, ref).cpu().numpy()
+        b = plan.execute(src, ref).cpu().numpy()
+        assert synth.sha256(a) == synth.sha256(b)
+        c = plan.execute_vagg(src, ref).cpu().numpy()
+        d = plan.execute_vagg(src, ref).cpu().numpy()
+        assert synth.sha256(c) == synth.sha256(d)
+
+
+# ---------------------------------------------------- mixed-precision filter
+# The mixed filter evaluates the 3-D DCT pair as FP16 matrix products with
+# FP32 accumulation: block pixels and intermediate coefficients are rounded
+# to FP16 (relative 2^-11); shrinkage, weights and aggregation stay FP32.
+# On the same group lists its per-slot estimates num/den agree with the FP32
+# filter's to that rounding, apart from coefficients that sit on the hard
+# threshold: a flipped coefficient c moves a pixel by at most 8 |c| / 4096
+# (0.015 at sigma 15), which bounds the max-abs tolerance below.
+
+def _slot_estimates(stack):
+    num, den = stack[:, :, :, 0], stack[:, :, :, 1]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return num / den, den > 0
+
+
+@pytest.mark.parametrize("extractor_exp", [0, 10])
+@pytest.mark.parametrize("case", CASES, ids=_ids())
+def test_mixed_filter_close_to_fp32(case, extractor_exp):
+    core = _gpu()
+    H, W, N, P, chroma, prm = case
+    prm = dict(prm, extractor_exp=extractor_exp)
+    r = prm["radius"]
+    _, noisy = synth.make_clip(H, W, N, P, noise_sd=prm["sigma"][0], seed=5, motion=True, yuv=chroma)
+    src = torch.from_numpy(noisy).cuda()
+    basic = None
+    # empirical bounds with headroom; threshold flips grow with sigma
+    max_abs = 0.03 * max(prm["sigma"]) / 15.0
+    max_rms = 1e-3 * max(prm["sigma"]) / 15.0
+    for final in (False, True):
+        p32 = _plan(core, W, H, prm, final, chroma)
+        pmx = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], r, prm["ps_num"],
+                        prm["ps_range"], chroma=chroma, final=final, extractor_exp=extractor_exp,
+                        precision="mixed")
+        assert pmx.uses_mixed_filter()
+        pmx.select_kernels("fp32", "mixed")  # same group lists as the FP32 plan
+        assert pmx.uses_mixed_filter() and not pmx.uses_mixed_kernel()
+        s32 = p32.execute(src, basic).cpu().numpy()
+        smx = pmx.execute(src, basic).cpu().numpy()
+        e32, m32 = _slot_estimates(s32)
+        emx, mmx = _slot_estimates(smx)
+        np.testing.assert_array_equal(mmx, m32)
+        d = np.abs(emx - e32)[m32]
+        rms = float(np.sqrt(np.mean(d.astype(np.float64) ** 2)))
+        print(f"final={final}: rms {rms:.2e} max {d.max():.2e}")
+        assert rms < max_rms and d.max() < max_abs
+        if not final:
+            est = core.vaggregate(torch.from_numpy(s32).cuda(), r, N) if r > 0 else \
+                core.aggregate(torch.from_numpy(s32).cuda())
+            est = est.cpu().numpy()
+            basic = torch.from_numpy(np.where(np.isfinite(est), est, noisy).astype(np.float32)).cuda()
+
+
+@pytest.mark.parametrize("kind", ["x255", "offset1000"])
+def test_mixed_filter_wide_range(kind):
+    """Inputs far outside [0, 1] (a 0-255 scale with sigma scaled alike, or a
+    large offset): the mixed filter works on each group's residual about an
+    offset, scaled by a power of two, so FP16 neither overflows nor loses
+    the residual's precision. Both filters are compared with the FP64 oracle
+    on the same input: the mixed filter's error stays at its FP16 level
+    relative to the scale (and, with the offset, below the FP32 filter's,
+    whose small coefficients lose digits to the offset)."""
+    core = _gpu()
+    H, W, N = 48, 64, 3
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=13, motion=True)
+    scale, off = (255.0, 0.0) if kind == "x255" else (1.0, 1000.0)
+    x = (noisy * np.float32(scale) + np.float32(off)).astype(np.float32)
+    src = torch.from_numpy(x).cuda()
+    prm = dict(block_step=4, bm_range=5, radius=1, ps_num=2, ps_range=3)
+    oprm = dict(prm, sigma=[15.0 * scale], extractor_exp=0)
+    basic = None
+    for final in (False, True):
+        p32 = core.Plan(W, H, 15.0 * scale, final=final, **prm)
+        pmx = core.Plan(W, H, 15.0 * scale, final=final, precision="mixed", **prm)
+        pmx.select_kernels("fp32", "mixed")
+        s32 = p32.execute(src, basic).cpu().numpy()
+        smx = pmx.execute(src, basic).cpu().numpy()
+        s64, _ = _oracle_stacks(x, None if basic is None else basic.cpu().numpy(), oprm, final, False,
+                                dtype=np.float64)
+        e32, m32 = _slot_estimates(s32)
+        emx, mmx = _slot_estimates(smx)
+        e64, m64 = _slot_estimates(s64)
+        np.testing.assert_array_equal(mmx, m32)
+        assert np.isfinite(emx[mmx]).all()
+        m = m32 & m64
+        err32 = float(np.sqrt(np.mean((e32[m] - e64[m]) ** 2))) / scale
+        errmx = float(np.sqrt(np.mean((emx[m] - e64[m]) ** 2))) / scale
+        print(f"{kind} final={final}: rms vs fp64: fp32 {err32:.2e} mixed {errmx:.2e}")
+        assert errmx < max(1e-3, 1.5 * err32)
+        if not final:
+            basic = core.vaggregate(torch.from_numpy(s32).cuda(), 1, N)
+
+
+def test_kernel_selection():
+    core = _gpu()
+    p32 = core.Plan(64, 48, 10.0)
+    assert not p32.uses_mixed_kernel() and not p32.uses_mixed_filter()
+    with pytest.raises(core.C.BM3DError, match="precision MIXED"):
+        p32.select_kernels("fp32", "mixed")
+    p32.select_kernels("fp32", "fp32")
+    pmx = core.Plan(64, 48, 10.0, precision="mixed")
+    assert pmx.uses_mixed_kernel() and pmx.uses_mixed_filter()
+    pmx.select_kernels("mixed", "fp32")
+    assert pmx.uses_mixed_kernel() and not pmx.uses_mixed_filter()
+    with pytest.raises(ValueError):
+        pmx.select_kernels("fp16", "fp32")
+    wide = core.Plan(64, 48, 10.0, bm_range=40, precision="mixed")
+    assert not wide.uses_mixed_kernel() and wide.uses_mixed_filter()
+    with pytest.raises(core.C.BM3DError, match="too large"):
+        wide.select_kernels("mixed", "mixed")
 
 
 # ------------------------------------------------ BM3D + VAggregate fused

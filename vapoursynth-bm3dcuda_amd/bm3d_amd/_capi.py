@@ -97,6 +97,10 @@ def lib():
     L.bm3d_num_blocks.restype = c_ll
     L.bm3d_plan_matching_kernel.argtypes = [P]
     L.bm3d_plan_matching_kernel.restype = c_int
+    L.bm3d_plan_filter_kernel.argtypes = [P]
+    L.bm3d_plan_filter_kernel.restype = c_int
+    L.bm3d_plan_select_kernels.argtypes = [P, c_int, c_int]
+    L.bm3d_plan_select_kernels.restype = c_int
     L.bm3d_execute.argtypes = [P, P, c_ll, c_ll, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
                                c_int, c_int, P]
     L.bm3d_execute.restype = c_int
@@ -156,7 +160,7 @@ def check(rc):
 EXPORTS = (
     "bm3d_plan_create", "bm3d_plan_destroy", "bm3d_stack_size", "bm3d_execute", "bm3d_execute_vagg",
     "bm3d_execute_window", "bm3d_aggregate", "bm3d_vaggregate", "bm3d_match",
-    "bm3d_num_blocks", "bm3d_plan_matching_kernel", "bm3d_last_error", "bm3d_version", "bm3d_profile_enable",
+    "bm3d_num_blocks", "bm3d_plan_matching_kernel", "bm3d_plan_filter_kernel", "bm3d_plan_select_kernels", "bm3d_last_error", "bm3d_version", "bm3d_profile_enable",
     "bm3d_profile_reset", "bm3d_profile_read", "bm3d_count_candidates",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
     "bm3d_filter_get_frames", "bm3d_filter_destroy",

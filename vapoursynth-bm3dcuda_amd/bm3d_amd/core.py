@@ -121,6 +121,17 @@ class Plan:
         """True when block matching runs on the mixed-precision kernel."""
         return bool(C.lib().bm3d_plan_matching_kernel(self._h))
 
+    def uses_mixed_filter(self):
+        """True when collaborative filtering runs on the mixed-precision kernel."""
+        return bool(C.lib().bm3d_plan_filter_kernel(self._h))
+
+    def select_kernels(self, matching, filter):
+        """Selects the matching and filter kernels: each "fp32" or "mixed"."""
+        codes = {"fp32": 0, "mixed": 1}
+        if matching not in codes or filter not in codes:
+            raise ValueError("kernel must be 'fp32' or 'mixed'")
+        C.check(C.lib().bm3d_plan_select_kernels(self._h, codes[matching], codes[filter]))
+
     def count_candidates(self, src, ref=None, first=0, count=None, stream=None):
         """Exact number of in-plane block-matching candidates for the frames."""
         _check_clip(src, "clip")

@@ -100,12 +100,18 @@ struct bm3d_plan {
     uint32_t* groups = nullptr;
     float* corner = nullptr;
     int* scratch = nullptr;
-    float fx_unit = 0.0f;  // weighted-sum fixed-point unit
+    // fixed-point unit of the accumulation regions: 2^fx_exp, scaled by the
+    // input's range (measured into *range per call) unless extractor_exp is set
+    int fx_exp = 0;
+    float* range = nullptr;
     int cap_frames = 0;
     // mixed-precision matching kernel (precision MIXED)
-    bool use_dx = false;
+    bool use_dx = false, dx_ok = false;
     int tile_hp2 = 0;
     size_t dx_lds_bytes = 0;
+    // mixed-precision filter kernel (precision MIXED): per-lane transform operands
+    bool use_mx_filter = false;
+    void* mx_ops = nullptr;
 };
 
 namespace {
@@ -143,8 +149,11 @@ void dx_geometry(bm3d_plan* p, int max_lds) {
     const size_t bytes = (size_t)2 * d.block_step * p->tile_wq * p->tile_hp2 * 4;
     if (bytes > (size_t)max_lds) return;
     p->dx_lds_bytes = bytes;
-    p->use_dx = true;
+    p->use_dx = p->dx_ok = true;
 }
+
+// Work-group size of the mixed filter kernel.
+constexpr int kMxFilterWaves = 4;
 
 // LDS bytes of the filter kernel for a tile of tbx x tby blocks and margin rm.
 size_t filter_lds(const bm3d_plan* p, int tbx, int tby, int rm, int* rw, int* rh) {
@@ -220,6 +229,20 @@ hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long l
     return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
 }
 
+// Range pointer of the fixed-point unit (null: the unit is fixed).
+const float* fx_range(const bm3d_plan* p) { return p->d.extractor_exp ? nullptr : p->range; }
+
+// Measures the largest finite |sample| of the processed planes of frames
+// lo .. hi of src into p->range (the fixed-point unit of every evaluation
+// and aggregation of the call follows from it).
+hipError_t measure_range(bm3d_plan* p, const bm3d_clip* src, int lo, int hi, hipStream_t stream) {
+    if (!fx_range(p)) return hipSuccess;
+    hipError_t e = bm3d::launch_zero(p->range, 1, 0, 1, stream);
+    if (e != hipSuccess) return e;
+    return bm3d::launch_absmax(src->data + (size_t)lo * src->frame_stride, src->frame_stride, src->plane_stride,
+                               p->nplanes, p->d.width, p->d.height, p->d.stride, hi - lo + 1, p->range, stream);
+}
+
 // Block matching and filtering of frames n0 .. n0 + nf - 1: group lists,
 // the tile accumulation regions (scratch) and the corner weight maps. With
 // ring > 0, frame n's regions and corner maps go to index n % ring of the
@@ -270,11 +293,18 @@ hipError_t evaluate(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, in
     fa.tbx = p->ftbx; fa.tby = p->ftby; fa.ntx = p->fntx; fa.nty = p->fnty;
     fa.rm = p->frm; fa.rw = p->frw; fa.rh = p->frh;
     fa.scratch = p->scratch;
-    fa.fx_scale = 1.0f / p->fx_unit;
+    fa.fx_range = fx_range(p);
+    fa.fx_exp = p->fx_exp;
     fa.ring = ring;
     {
         ProfScope ps(1, stream);
-        e = bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream);
+        if (p->use_mx_filter) {
+            fa.mx_ops = p->mx_ops;
+            e = bm3d::launch_filter_mx(fa, d.final_ != 0, d.chroma != 0, nf, kMxFilterWaves, p->filter_lds_bytes,
+                                       stream);
+        } else {
+            e = bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream);
+        }
     }
     return e;
 }
@@ -325,7 +355,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     // Fixed-point unit of the LDS accumulation: with extractor 2^e, every
     // pre-rounded value in (-E/2, 2E) is a multiple of 2^(e-24), so sums are
     // exact; otherwise 2^-22 (1.2e-7 rounding, sums up to +-512).
-    p->fx_unit = d.extractor_exp ? std::ldexp(1.0f, d.extractor_exp - 24) : std::ldexp(1.0f, -22);
+    p->fx_exp = d.extractor_exp ? d.extractor_exp - 24 : -22;
     p->nplanes = p->d.chroma ? 3 : 1;
     p->tw = 2 * d.radius + 1;
     p->nbx = count_positions(d.width, d.block_step);
@@ -380,6 +410,24 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
             return hip_fail(e, "hipFuncSetAttribute");
         }
     }
+    e = hipMalloc(&p->range, sizeof(float));
+    if (e != hipSuccess) {
+        bm3d_plan_destroy(p);
+        return hip_fail(e, "hipMalloc");
+    }
+    if (d.precision == BM3D_PRECISION_MIXED) {
+        std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);
+        bm3d::build_filter_mx_ops(ops.data());
+        const size_t bytes = ops.size() * sizeof(uint32_t);
+        e = hipMalloc(&p->mx_ops, bytes);
+        if (e == hipSuccess) e = hipMemcpy(p->mx_ops, ops.data(), bytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess && p->filter_lds_bytes > 65536) e = bm3d::set_filter_mx_lds_limit(p->filter_lds_bytes);
+        if (e != hipSuccess) {
+            bm3d_plan_destroy(p);
+            return hip_fail(e, "mixed filter set-up");
+        }
+        p->use_mx_filter = true;
+    }
     *out = p;
     return BM3D_OK;
 }
@@ -389,6 +437,8 @@ void bm3d_plan_destroy(bm3d_plan* p) {
     if (p->groups) (void)hipFree(p->groups);
     if (p->corner) (void)hipFree(p->corner);
     if (p->scratch) (void)hipFree(p->scratch);
+    if (p->mx_ops) (void)hipFree(p->mx_ops);
+    if (p->range) (void)hipFree(p->range);
     delete p;
 }
 
@@ -419,6 +469,8 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
     if (int rc = ensure_scratch(p, batch)) return rc;
 
     const bm3d_desc& d = p->d;
+    BM3D_HIP(measure_range(p, src, std::max(first_frame - d.radius, 0),
+                           std::min(first_frame + count - 1 + d.radius, src->num_frames - 1), stream));
     for (int f0 = 0; f0 < count; f0 += batch) {
         const int nf = std::min(batch, count - f0);
         const int n0 = first_frame + f0;
@@ -440,7 +492,8 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
 
         bm3d::FinalizeArgs fz{};
         fz.scratch = p->scratch;
-        fz.fx_unit = p->fx_unit;
+        fz.fx_range = fx_range(p);
+        fz.fx_exp = p->fx_exp;
         fz.res = res;
         fz.res_frame_stride = res_frame_stride;
         fz.res_plane_stride = res_plane_stride;
@@ -512,6 +565,22 @@ int bm3d_profile_read(int kernel, double* total_ms, long long* launches) {
 }
 
 int bm3d_plan_matching_kernel(const bm3d_plan* p) { return p && p->use_dx ? 1 : 0; }
+
+int bm3d_plan_filter_kernel(const bm3d_plan* p) { return p && p->use_mx_filter ? 1 : 0; }
+
+int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
+    g_last_error.clear();
+    if (!p) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (matching < 0 || matching > 1 || filter < 0 || filter > 1)
+        return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision)");
+    if ((matching || filter) && p->d.precision != BM3D_PRECISION_MIXED)
+        return fail(BM3D_EINVAL, "BM3D: mixed-precision kernels need a plan created with precision MIXED");
+    if (matching && !p->dx_ok)
+        return fail(BM3D_EUNSUPPORTED, "BM3D: search windows too large for the mixed-precision matching kernel");
+    p->use_dx = matching != 0;
+    p->use_mx_filter = filter != 0;
+    return BM3D_OK;
+}
 
 int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
                           unsigned long long* ncand, void* stream) {
@@ -586,7 +655,8 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     va.corner = p->corner;
     va.corner_frame = p->corner_frame;
     va.ring = ring;
-    va.fx_unit = p->fx_unit;
+    va.fx_range = fx_range(p);
+    va.fx_exp = p->fx_exp;
     va.dst_frame_stride = dst_frame_stride;
     va.dst_plane_stride = dst_plane_stride;
     va.dst_stride = dst_stride;
@@ -601,6 +671,7 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     va.bs = d.block_step; va.tbx = p->ftbx; va.tby = p->ftby; va.ntx = p->fntx; va.nty = p->fnty;
     va.rm = p->frm; va.rw = p->frw; va.rh = p->frh;
 
+    BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), stream));
     int next_out = first_frame;
     for (int b0 = mlo; b0 <= mhi;) {
         const int b1 = std::min(b0 + batch, mhi + 1);

@@ -1,5 +1,5 @@
 // Device helpers shared by the gfx950 kernels (bm3d_kernels.hip,
-// bm3d_match_mx.hip). Internal.
+// bm3d_match_dx.hip, bm3d_filter_mx.hip). Internal.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -75,6 +75,19 @@ template <int kCtrl>
 __device__ __forceinline__ int dpp_mov(int v) {
     return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
 }
+// Exponent of the fixed-point unit of the accumulation regions. With
+// range null, fx_exp (extractor_exp - 24: pre-rounded values are whole
+// multiples of the unit). Otherwise fx_exp (-22) plus floor(log2 M) for
+// the largest finite |sample| M >= 2 of the call's input: a region pixel's
+// sum of weighted estimates (weights <= 1, a few hundred members at most)
+// then has the int32 headroom of 2^31 units >= 256 M whatever the scale of
+// the samples, and every input within (-2, 2) keeps the unit 2^-22.
+__device__ __forceinline__ int fx_exponent(const float* range, int fx_exp) {
+    if (!range) return fx_exp;
+    const float m = *range;
+    return m >= 2.0f ? fx_exp + ilogbf(m) : fx_exp;
+}
+
 // xor1 / xor2 are quad permutations; xor4 is a half-row mirror followed by a
 // quad reversal (7 - k, then 3 - (k mod 4) within the quad, gives k ^ 4).
 template <typename T> __device__ __forceinline__ T lane_xor1(T v) { return dpp_mov<0xB1>(v); }

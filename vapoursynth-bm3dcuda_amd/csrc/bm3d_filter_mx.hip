@@ -1,0 +1,379 @@
+// Mixed-precision collaborative filtering (BM3D_PRECISION_MIXED).
+//
+// The filter of source/kernel.cu:649-756 (hard threshold 208-281, Wiener
+// 284-363) with the separable 3-D DCT evaluated as FP16 matrix products on
+// the MFMA pipe (FP32 accumulation); shrinkage, weights and aggregation stay
+// in FP32 and use the FP32 kernel's accumulation scheme (int32 fixed-point
+// LDS regions per work-group tile, corner weight map), so the consumers of
+// the regions (bm3d_finalize, bm3d_vagg_regions) are shared.
+//
+// A wave filters two groups (16 member blocks b) per step. With D the 8-point
+// DCT-II (4 x orthonormal) and K = D (x) D acting on a block's 64 pixels:
+//
+//   Y^T = X^T K^T      16 blocks x 64 coefficients  (v_mfma_f32_16x16x32_f16)
+//   Z^T = Dbd Y^T      along the members            (v_mfma_f32_16x16x16_f16)
+//   shrink Z in FP32
+//   W   = Z Dbd        members back                 (v_mfma_f32_16x16x16_f16)
+//   X'  = K^T W        pixels back                  (v_mfma_f32_16x16x32_f16)
+//
+// where Dbd = diag(D, D) keeps the two groups apart. Each product's result
+// registers are the next product's operand registers as they stand (the
+// output layout of one MFMA is the B layout of the next, or the A layout of
+// the transposed matrix), so no lane exchange or LDS staging is needed
+// between the four stages. The operand registers of K, K^T and Dbd are
+// precomputed per lane on the host (build_filter_mx_ops).
+//
+// Lane l = 16 q + c. Loads: lane (q, c) reads rows q and q + 4 of block c
+// (member c % 8 of group c / 8). Coefficients: lane (q, c) holds Z[m][u]
+// for member frequency m = 4 q + i (group q / 2) and 2-D coefficient
+// u = 16 t + c, i, t = 0..3. Estimates: lane (q, c) holds pixels
+// (row 2 T + q / 2, column 4 (q % 2) + i) of block c, T, i = 0..3.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "bm3d_device.h"
+
+namespace bm3d {
+
+namespace {
+
+using namespace dev;
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ h8 cvt8(f4u lo, f4u hi) {
+    return h8{(_Float16)lo[0], (_Float16)lo[1], (_Float16)lo[2], (_Float16)lo[3],
+              (_Float16)hi[0], (_Float16)hi[1], (_Float16)hi[2], (_Float16)hi[3]};
+}
+__device__ __forceinline__ h4 cvt4(f4 v) {
+    return h4{(_Float16)v[0], (_Float16)v[1], (_Float16)v[2], (_Float16)v[3]};
+}
+__device__ __forceinline__ f4 mfma32(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// Per-lane transform operands (kMxOpVectors uint4 per lane, lane-interleaved).
+struct MxOps {
+    h8 kf[4][2];  // K^T as the B operand of Y^T = X^T K^T: [tile t][k-step s]
+    h8 ki[4][2];  // K^T as the A operand of X' = K^T W: [tile T][k-step s]
+    h4 dA;        // Dbd as the A operand (forward along members)
+    h4 dB;        // Dbd as the B operand (inverse along members)
+
+    __device__ __forceinline__ void load(const uint4* __restrict__ ops, int lane) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) kf[v >> 1][v & 1] = __builtin_bit_cast(h8, ops[v * 64 + lane]);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) ki[v >> 1][v & 1] = __builtin_bit_cast(h8, ops[(8 + v) * 64 + lane]);
+        const uint4 d = ops[16 * 64 + lane];
+        dA = __builtin_bit_cast(h4, uint2{d.x, d.y});
+        dB = __builtin_bit_cast(h4, uint2{d.z, d.w});
+    }
+};
+
+// Largest |v| over the 64 lanes (every lane gets it).
+__device__ __forceinline__ float wave_max_abs(float m) {
+    m = fmaxf(m, lane_xor1(m));
+    m = fmaxf(m, lane_xor2(m));
+    m = fmaxf(m, dpp_mov<0x141>(m));
+    m = fmaxf(m, dpp_mov<0x140>(m));
+    const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 0));
+    const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 16));
+    const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 32));
+    const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// Forward 3-D transform of the two groups whose member blocks start at mp
+// (lane (q, c): block c): z[t][i] = Z[4 q + i][16 t + c] of the residual
+// (X - mu) / s. mu is the group's offset (top-left pixel of its first
+// member, the reference block; returned for the lane's block's group) and s
+// a power of two (returned, shared by both groups) that keeps every
+// residual within +-4, so no FP16 operand of the forward or inverse
+// products can overflow: |Y| <= 128 x 4, |Z| <= 8 sqrt(2) |Y|, and W, the
+// member inverse of shrunk Z, has 2-norm at most 16 x the 2-norm of Y's
+// 8 members (|W| < 23200). A constant
+// offset only moves the DC coefficient, which both shrinkage rules pass
+// unchanged, so mu is added back exactly after the inverse.
+__device__ __forceinline__ void forward_mx(f4 (&z)[4], float& mu, float& s, const MxOps& o,
+                                           const float* __restrict__ mp, int stride, int q, int c) {
+    const float* r0 = mp + (size_t)q * stride;
+    const float* r1 = r0 + (size_t)4 * stride;
+    f4u v[4] = {*reinterpret_cast<const f4u*>(r0), *reinterpret_cast<const f4u*>(r0 + 4),
+                *reinterpret_cast<const f4u*>(r1), *reinterpret_cast<const f4u*>(r1 + 4)};
+    const float mu0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[0][0]), 0));
+    const float mu1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[0][0]), 8));
+    mu = (c >> 3) ? mu1 : mu0;
+    float m = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[h][e] = v[h][e] - mu;
+            m = fmaxf(m, fabsf(v[h][e]));
+        }
+    const float g = wave_max_abs(m);
+    s = 1.0f;
+    if (g > 4.0f && g <= kFltMax) {  // rare: residuals beyond +-4 (not NaN / inf)
+        const int k = ilogbf(g) - 1;  // g / 2^k < 4
+        s = ldexpf(1.0f, k);
+        const float inv = ldexpf(1.0f, -k);
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[h][e] = v[h][e] * inv;
+    }
+    const h8 a0 = cvt8(v[0], v[1]);
+    const h8 a1 = cvt8(v[2], v[3]);
+    const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f4 y = mfma32(a0, o.kf[t][0], zero);
+        y = mfma32(a1, o.kf[t][1], y);
+        z[t] = mfma16(o.dA, cvt4(y), zero);
+    }
+}
+
+// Inverse 3-D transform of shrunk coefficients z (layout of forward_mx):
+// x[T][i] = estimate at pixel (2 T + q / 2, 4 (q % 2) + i) of block c,
+// scaled by 4096 (the transform pair's gain).
+__device__ __forceinline__ void inverse_mx(f4 (&x)[4], const MxOps& o, const f4 (&z)[4]) {
+    const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    h4 w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[t] = cvt4(mfma16(cvt4(z[t]), o.dB, zero));
+    const h8 b0 = h8{w[0][0], w[0][1], w[0][2], w[0][3], w[1][0], w[1][1], w[1][2], w[1][3]};
+    const h8 b1 = h8{w[2][0], w[2][1], w[2][2], w[2][3], w[3][0], w[3][1], w[3][2], w[3][3]};
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+        f4 v = mfma32(o.ki[T][0], b0, zero);
+        x[T] = mfma32(o.ki[T][1], b1, v);
+    }
+}
+
+// Sum over the 16 lanes of each row of the wave (every lane gets its row's sum).
+__device__ __forceinline__ float row_sum16(float s) {
+    s = s + lane_xor1(s);
+    s = s + lane_xor2(s);
+    s = s + dpp_mov<0x141>(s);  // half-row mirror: quad 0 <-> quad 1 of each 8 lanes
+    s = s + dpp_mov<0x140>(s);  // row mirror: lanes 0-7 <-> 8-15
+    return s;
+}
+
+// Grid (ntx, nty, frames), blockDim.x = 64 x waves. Same tile, region and
+// scratch layout as bm3d_filter (bm3d_kernels.hip).
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
+    extern __shared__ float lds[];
+    const int tw = 2 * a.radius + 1;
+    const int region = tw * a.rh * a.rw;
+    int* acc = reinterpret_cast<int*>(lds);
+    const int nwaves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int q = lane >> 4, c = lane & 15;
+    const int tx = blockIdx.x, ty = blockIdx.y, f = blockIdx.z;
+    const int n = a.first_frame + f;
+    const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
+    const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
+    const int ngroups = a.tbx * a.tby;
+    const float E = a.extractor;
+    const int nplanes = kChroma ? 3 : 1;
+    const int slot = a.ring > 0 ? n % a.ring : f;
+    const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
+
+    MxOps o;
+    o.load(reinterpret_cast<const uint4*>(a.mx_ops), lane);
+
+    for (int p = 0; p < nplanes; ++p) {
+        const float sigma = a.sigma[p];
+        if (kChroma && sigma < kFltEpsilon) continue;  // source/kernel.cu:657-668
+        for (int i = threadIdx.x; i < region; i += blockDim.x) acc[i] = 0;
+        __syncthreads();
+        const float* src = a.src + (size_t)p * a.plane_stride;
+        float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
+        float* corner = a.corner + (size_t)slot * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
+
+        for (int base = 0; base < ngroups; base += nwaves * 2) {
+            const int li = base + wave * 2 + (c >> 3);
+            const int bx = tx * a.tbx + li % a.tbx;
+            const int by = ty * a.tby + li / a.tbx;
+            const bool active = li < ngroups && bx < a.nbx && by < a.nby;
+            const int blk = active ? by * a.nbx + bx : 0;
+            const uint32_t pk = a.groups[((size_t)f * a.nblocks + blk) * 8 + (c & 7)];
+            // positions come from block matching and are in range; the clamps
+            // only keep a corrupted list from addressing outside the planes
+            const int z = min(pos_z(pk), 2 * a.radius);
+            const int px = min(pos_x(pk), a.W - 8);
+            const int py = min(pos_y(pk), a.H - 8);
+            const int fi = clampi(n - a.radius + z, 0, a.num_frames - 1);
+            const float* mp = src + (size_t)fi * a.frame_stride + (size_t)py * a.stride + px;
+
+            f4 zc[4];
+            float w0, w1, mu, sc;
+            if constexpr (kFinal) {
+                // Wiener shrinkage (source/kernel.cu:284-326): coefficients
+                // from the basic estimate's group, applied to the noisy group.
+                // The DC coefficient is 1, so the offsets pass unchanged.
+                f4 cf[4];
+                float mub, sb;
+                forward_mx(cf, mub, sb, o, mp + (a.ref - a.src), a.stride, q, c);
+                const float s2 = (sigma * sigma) / (sb * sb);  // sigma^2 in units of the scaled residual
+                float sum = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float sq = cf[t][i] * cf[t][i];
+                        float coef = sq * __builtin_amdgcn_rcpf(sq + s2);
+                        if (t == 0 && i == 0) coef = (c == 0 && (q & 1) == 0) ? 1.0f : coef;
+                        sum = fma_(coef, coef, sum);
+                        cf[t][i] = coef;
+                    }
+                sum = row_sum16(sum);
+                // rows 0-1 hold group 0's coefficients, rows 2-3 group 1's
+                const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sum), 0));
+                const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sum), 16));
+                const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sum), 32));
+                const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sum), 48));
+                w0 = 1.0f / (r0 + r1);
+                w1 = 1.0f / (r2 + r3);
+                forward_mx(zc, mu, sc, o, mp, a.stride, q, c);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) zc[t][i] = zc[t][i] * cf[t][i];
+            } else {
+                // hard thresholding (source/kernel.cu:208-252); the DC
+                // coefficient is compared with 0 instead of sigma
+                forward_mx(zc, mu, sc, o, mp, a.stride, q, c);
+                const float thr_ac = sigma / sc;  // sigma in units of the scaled residual
+                int kept0 = 0, kept1 = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float v = zc[t][i];
+                        const float thr = (t == 0 && i == 0 && c == 0 && (q & 1) == 0) ? 0.0f : thr_ac;
+                        const bool keep = fabsf(v) >= thr;
+                        const unsigned long long m = __ballot(keep);
+                        kept0 += __builtin_popcount((uint32_t)m);
+                        kept1 += __builtin_popcount((uint32_t)(m >> 32));
+                        zc[t][i] = keep ? v : 0.0f;
+                    }
+                w0 = 1.0f / (float)kept0;
+                w1 = 1.0f / (float)kept1;
+            }
+            f4 x[4];
+            inverse_mx(x, o, zc);
+
+            if (active) {
+                // aggregation with pre-rounding (source/kernel.cu:717-756);
+                // the transform gain 4096 is divided out with the weight
+                const float w = (c >> 3) ? w1 : w0;
+                const float ws = w * (sc * (1.0f / 4096.0f));  // estimate = x s / 4096 + mu
+                const float wm = w * mu;
+                const int lx = px - rx0, ly = py - ry0;
+                const int r = q >> 1, col = 4 * (q & 1);
+                if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
+                    int* qa = acc + (z * a.rh + ly + r) * a.rw + lx + col;
+#pragma unroll
+                    for (int T = 0; T < 4; ++T)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            float val = fma_(ws, x[T][i], wm);
+                            val = (val + E) - E;
+                            atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(val * fx_scale));
+                        }
+                } else if (a.res) {
+                    float* qr = res + (size_t)(2 * z) * a.plane_size + (size_t)(py + r) * a.stride + px + col;
+#pragma unroll
+                    for (int T = 0; T < 4; ++T)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            float val = fma_(ws, x[T][i], wm);
+                            val = (val + E) - E;
+                            unsafeAtomicAdd(qr + (size_t)(2 * T) * a.stride + i, val);
+                        }
+                }
+                if (q == 0) {
+                    const float wr = (w + E) - E;
+                    unsafeAtomicAdd(corner + (size_t)z * a.plane_size + (size_t)py * a.stride + px, wr);
+                }
+            }
+        }
+        __syncthreads();
+        int* out = a.scratch + ((((size_t)slot * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
+        for (int i = threadIdx.x; i < region; i += blockDim.x) out[i] = acc[i];
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// 8-point DCT-II, 4 x orthonormal: D[0][x] = sqrt(2), D[m][x] = 2 cos((2x + 1) m pi / 16).
+static double dct_entry(int m, int x) {
+    const double pi = 3.14159265358979323846264338327950288;
+    return m == 0 ? ::sqrt(2.0) : 2.0 * ::cos((2 * x + 1) * m * pi / 16.0);
+}
+
+void build_filter_mx_ops(uint32_t* out) {
+    auto h = [](double v) -> uint16_t { return __builtin_bit_cast(uint16_t, (_Float16)v); };
+    // K[u][p] = D[u / 8][p / 8] D[u % 8][p % 8] for coefficient u and pixel p
+    auto K = [](int u, int p) { return dct_entry(u >> 3, p >> 3) * dct_entry(u & 7, p & 7); };
+    auto Dbd = [](int m, int b) { return (m >> 3) == (b >> 3) ? dct_entry(m & 7, b & 7) : 0.0; };
+    for (int lane = 0; lane < 64; ++lane) {
+        const int q = lane >> 4, c = lane & 15;
+        uint16_t e[kMxOpVectors][8];
+        for (int v = 0; v < 8; ++v) {
+            const int t = v >> 1, s = v & 1;
+            // B[k = 8 q + kk][n = c] of Y^T = X^T K^T: pixel (row q + 4 s, column kk), coefficient 16 t + c
+            for (int kk = 0; kk < 8; ++kk) e[v][kk] = h(K(16 * t + c, (q + 4 * s) * 8 + kk));
+            // A[r = c][k = 8 q + j] of X' = K^T W: pixel 16 T + c, coefficient 32 s + 16 (j / 4) + 4 q + j % 4
+            for (int j = 0; j < 8; ++j) e[8 + v][j] = h(K(32 * s + 16 * (j >> 2) + 4 * q + (j & 3), 16 * t + c));
+        }
+        for (int i = 0; i < 4; ++i) {
+            e[16][i] = h(Dbd(c, 4 * q + i));      // A[m = c][k = 4 q + i]
+            e[16][4 + i] = h(Dbd(4 * q + i, c));  // B[k = 4 q + i][n = c]
+        }
+        for (int v = 0; v < kMxOpVectors; ++v)
+            for (int w = 0; w < 4; ++w)
+                out[(v * 64 + lane) * 4 + w] = (uint32_t)e[v][2 * w] | ((uint32_t)e[v][2 * w + 1] << 16);
+    }
+}
+
+hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int frames, int waves, size_t lds_bytes,
+                            hipStream_t s) {
+    dim3 grid(a.ntx, a.nty, frames), block(64 * waves);
+    if (final_) {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<true, true>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter_mx<true, false>), grid, block, lds_bytes, s, a);
+    } else {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<false, true>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter_mx<false, false>), grid, block, lds_bytes, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t set_filter_mx_lds_limit(size_t bytes) {
+    const void* ks[] = {
+        reinterpret_cast<const void*>(&bm3d_filter_mx<false, false>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<false, true>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<true, false>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<true, true>),
+    };
+    for (const void* k : ks) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace bm3d

@@ -55,13 +55,16 @@ struct FilterArgs {
     int tbx, tby, ntx, nty;  // work-group tile of reference blocks, tile grid
     int rm, rw, rh;          // accumulation region: margin, width, height
     int* scratch;            // [frames][nty][ntx][planes][2r+1][rh][rw], fixed point
-    float fx_scale;          // float -> fixed-point units (1 / fx_unit)
+    const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
+    int fx_exp;
     int ring;                // > 0: frame n's scratch and corner maps at index n % ring
+    const void* mx_ops;      // mixed filter: per-lane transform operands (build_filter_mx_ops)
 };
 
 struct FinalizeArgs {
     const int* scratch;
-    float fx_unit;           // value of one fixed-point unit
+    const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
+    int fx_exp;
     float* res;
     long long res_frame_stride, res_plane_stride;
     int W, H, stride;
@@ -102,7 +105,8 @@ struct VRegArgs {
     const float* corner;     // [ring][planes][2r+1][H][stride]
     long long corner_frame;  // floats per evaluation
     int ring;                // evaluation of frame m at index m % ring
-    float fx_unit;
+    const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
+    int fx_exp;
     float* dst;              // output frames [first_frame, first_frame + count)
     long long dst_frame_stride, dst_plane_stride;
     int dst_stride;
@@ -138,9 +142,20 @@ int dx_temporal_keys(int ps_num);
 hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
                          hipStream_t s);
 hipError_t set_filter_lds_limit(size_t bytes);
+// Mixed-precision filter (bm3d_filter_mx.hip): same arguments, tiles and
+// outputs as launch_filter; `waves` waves per work-group (1..8).
+constexpr int kMxOpVectors = 17;  // uint4 per lane of FilterArgs::mx_ops
+void build_filter_mx_ops(uint32_t* out);  // kMxOpVectors * 64 * 4 words, host
+hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int frames, int waves, size_t lds_bytes,
+                            hipStream_t s);
+hipError_t set_filter_mx_lds_limit(size_t bytes);
 hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s);
 // Zeroes `count` runs of `n` floats, `step` floats apart.
 hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s);
+// Largest finite |sample| of planes 0 .. nplanes-1 of `frames` frames from
+// clip, max-ed into *out (bit pattern compare; *out must start at zero).
+hipError_t launch_absmax(const float* clip, long long frame_stride, long long plane_stride, int nplanes, int W,
+                         int H, int stride, int frames, float* out, hipStream_t s);
 hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s);
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s);
 hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s);

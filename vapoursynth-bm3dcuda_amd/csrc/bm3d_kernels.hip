@@ -295,6 +295,7 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
     const int nplanes = kChroma ? 3 : 1;
+    const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
 
     for (int p = 0; p < nplanes; ++p) {
         const float sigma = a.sigma[p];
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
                         for (int j = 0; j < 8; ++j) {
                             float val = w * v[i][j];
                             val = (val + E) - E;
-                            atomicAdd(q + j * a.rw, (int)__builtin_rintf(val * a.fx_scale));
+                            atomicAdd(q + j * a.rw, (int)__builtin_rintf(val * fx_scale));
                         }
                     } else if (a.res) {
                         float* q = res + (size_t)(2 * z) * a.plane_size + (size_t)py * a.stride + px + k;
@@ -443,6 +444,31 @@ __global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t inc = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += inc) q[i] = 0.0f;
+}
+
+// Largest finite |sample| over the planes of `frames` frames: grid (rows,
+// frames * nplanes), 256 threads over the columns; one atomic per wave on
+// the float's bit pattern (non-negative floats order like their bits).
+__global__ __launch_bounds__(256) void bm3d_absmax(const float* __restrict__ clip, long long frame_stride,
+                                                   long long plane_stride, int nplanes, int W, int H, int stride,
+                                                   float* out) {
+    const int f = blockIdx.y / nplanes, p = blockIdx.y % nplanes;
+    const float* base = clip + (size_t)f * frame_stride + (size_t)p * plane_stride;
+    float m = 0.0f;
+    for (int y = blockIdx.x; y < H; y += gridDim.x) {
+        const float* row = base + (size_t)y * stride;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const float v = fabsf(row[x]);
+            m = v <= kFltMax ? fmaxf(m, v) : m;  // skips NaN and inf
+        }
+    }
+    m = fmaxf(m, lane_xor1(m));
+    m = fmaxf(m, lane_xor2(m));
+    m = fmaxf(m, dpp_mov<0x141>(m));
+    m = fmaxf(m, dpp_mov<0x140>(m));
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(m));
 }
 
 // Weighted-sum rows of the stacks: the integer sum of the tile regions
@@ -481,7 +507,7 @@ __global__ void bm3d_finalize(FinalizeArgs a) {
             }
         }
     }
-    num[0] = s + (float)isum * a.fx_unit;
+    num[0] = s + (float)isum * ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
 }
 
 // den[y][x] = sum over a, b in [0, 8) of corner[y - a][x - b]: each member
@@ -587,6 +613,7 @@ __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
     const int tw = 2 * r + 1;
     const int region = tw * a.rh * a.rw;
     const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
+    const float fx_unit = ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
     float num[2] = {0.0f, 0.0f}, den[2] = {0.0f, 0.0f};
     for (int i = 0; i < tw; ++i) {
         const int m = clampi(n - r + i, 0, a.num_frames - 1);
@@ -634,7 +661,7 @@ __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
                     isum += reg[(z * a.rh + ly) * a.rw + lx];
                 }
             }
-            num[h] = num[h] + (0.0f + (float)isum * a.fx_unit);
+            num[h] = num[h] + (0.0f + (float)isum * fx_unit);
             den[h] = den[h] + dv;
         }
     }
@@ -704,6 +731,14 @@ hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(bm3d_zero, dim3((unsigned)blocks, count), dim3(256), 0, s, p, n, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float* clip, long long frame_stride, long long plane_stride, int nplanes, int W,
+                         int H, int stride, int frames, float* out, hipStream_t s) {
+    dim3 grid(H < 64 ? H : 64, frames * nplanes);
+    hipLaunchKernelGGL(bm3d_absmax, grid, dim3(256), 0, s, clip, frame_stride, plane_stride, nplanes, W, H, stride,
+                       out);
     return hipGetLastError();
 }
 
