@@ -593,18 +593,48 @@ __global__ void bm3d_vaggregate(VAggArgs a) {
     a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] = num / den;
 }
 
-// VAggregate (source/source.cpp:834-836) of the stacks bm3d_finalize and
-// bm3d_den_box would write, computed from their sources: for each slot i of
-// output frame n, the weighted sum is the integer sum of the covering tile
-// regions times fx_unit (as bm3d_finalize) and the weight is the 8x8 box sum
-// of the corner map (as bm3d_den_box, same order); both are added in slot
-// order and divided (as bm3d_vaggregate). Output tile 64 x 8, two pixels per
-// thread. Grid: (ceil(W / 64), ceil(H / 8), count * nplanes).
+// Tiles whose accumulation regions cover pixel coordinate v along one axis:
+// the contiguous range [lo, hi] (origins min(t * span, limit) - rm are
+// non-decreasing in t), found with a float reciprocal and exact integer
+// corrections instead of integer divisions.
+__device__ __forceinline__ void covering_tiles(int v, int span, float inv_span, int limit, int rm, int rlen,
+                                               int ntiles, int& lo, int& hi) {
+    // hi: last t with min(t * span, limit) - rm <= v
+    const int u = v + rm;
+    int t = (int)((float)u * inv_span);
+    t += (t + 1) * span <= u ? 1 : 0;
+    t -= t * span > u ? 1 : 0;
+    hi = min(t, ntiles - 1);
+    if (limit - rm <= v) hi = ntiles - 1;  // the clamped last tile
+    // lo: first t with min(t * span, limit) - rm + rlen > v
+    const int w = v + rm - rlen + 1;  // need t * span >= w (or the clamped origin)
+    if (w <= 0) {
+        lo = 0;
+    } else {
+        int c = (int)((float)w * inv_span);
+        c += c * span < w ? 1 : 0;
+        c -= (c - 1) * span >= w ? 1 : 0;
+        lo = c;
+    }
+    while (lo > 0 && min((lo - 1) * span, limit) - rm + rlen > v) --lo;
+    while (lo <= hi && min(lo * span, limit) - rm + rlen <= v) ++lo;
+}
+
+// VAggregate of output frames first_frame .. first_frame + count - 1 straight
+// from the evaluations' tile regions and corner maps (the stacks are never
+// formed): for slot i = 0..2r (source/source.cpp:834-836), frame
+// m = clamp(n - r + i) holds the evaluation and z = clamp(2r - i, ...) the
+// slot; num = the regions' integer sum times the unit (as bm3d_finalize),
+// den = the corner map's 8 x 8 box sum (as bm3d_den_box, same order), both
+// added over i in order from 0, then num / den. Work-group: 64 x 8 pixels,
+// grid (ceil(W/64), ceil(H/8), count * nplanes).
 __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
+    __shared__ float cm[15][72];      // corner-map window: rows y0-7.., columns x0-7..
     __shared__ float rowsum[15][64];
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // 0..3
-    const int x = blockIdx.x * 64 + tx;
+    const int x0 = blockIdx.x * 64;
+    const int x = x0 + tx;
     const int y0 = blockIdx.y * 8;
     const int fo = blockIdx.z / a.nplanes, p = blockIdx.z % a.nplanes;
     if (!((a.plane_mask >> p) & 1u)) return;  // uniform per work-group
@@ -614,6 +644,17 @@ __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
     const int region = tw * a.rh * a.rw;
     const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
     const float fx_unit = ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
+    // covering tiles and region coordinates of this thread's two pixels
+    int xlo, xhi;
+    covering_tiles(min(x, a.W - 1), tw_, 1.0f / (float)tw_, a.W - 8, a.rm, a.rw, a.ntx, xlo, xhi);
+    const int lx0 = x - (min(xlo * tw_, a.W - 8) - a.rm);
+    int ylo[2], yhi[2], ly0[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = min(y0 + ty + 4 * h, a.H - 1);
+        covering_tiles(y, th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo[h], yhi[h]);
+        ly0[h] = y - (min(ylo[h] * th, a.H - 8) - a.rm);
+    }
     float num[2] = {0.0f, 0.0f}, den[2] = {0.0f, 0.0f};
     for (int i = 0; i < tw; ++i) {
         const int m = clampi(n - r + i, 0, a.num_frames - 1);
@@ -621,45 +662,34 @@ __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
         const int slot = m % a.ring;
         const float* cmap = a.corner + (size_t)slot * a.corner_frame + (size_t)(p * tw + z) * a.plane_size;
         __syncthreads();
+        for (int k = threadIdx.x; k < 15 * 71; k += 256) {
+            const int rr = k / 71, cc = k - rr * 71;
+            const int yy = y0 - 7 + rr, xx = x0 - 7 + cc;
+            cm[rr][cc] = (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) ? cmap[(size_t)yy * a.stride + xx] : 0.0f;
+        }
+        __syncthreads();
         for (int rr = ty; rr < 15; rr += 4) {
-            const int yy = y0 - 7 + rr;
             float s = 0.0f;
-            if (yy >= 0 && yy < a.H) {
-                const float* row = cmap + (size_t)yy * a.stride;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const int xx = x - b;
-                    s = s + ((xx >= 0 && xx < a.W) ? row[xx] : 0.0f);
-                }
-            }
+            for (int b = 0; b < 8; ++b) s = s + cm[rr][tx + 7 - b];
             rowsum[rr][tx] = s;
         }
         __syncthreads();
-        const int* scr = a.scratch + (size_t)slot * a.scratch_frame;
+        const int* scr = a.scratch + (size_t)slot * a.scratch_frame + (size_t)p * region + (size_t)z * a.rh * a.rw;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int jj = ty + 4 * h;
-            const int y = y0 + jj;
-            if (y >= a.H || x >= a.W) continue;
             float dv = 0.0f;
 #pragma unroll
             for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
             long long isum = 0;
-            const int ylo_n = y - a.rm - 7 - (a.tby - 1) * a.bs;
-            const int xlo_n = x - a.rm - 7 - (a.tbx - 1) * a.bs;
-            const int ty_lo = max((ylo_n < 0 ? 0 : ylo_n / th) - 1, 0);
-            const int ty_hi = min((y + a.rm) / th + 1, a.nty - 1);
-            const int tx_lo = max((xlo_n < 0 ? 0 : xlo_n / tw_) - 1, 0);
-            const int tx_hi = min((x + a.rm) / tw_ + 1, a.ntx - 1);
-            for (int tyy = ty_lo; tyy <= ty_hi; ++tyy) {
-                const int ly = y - (min(tyy * th, a.H - 8) - a.rm);
-                if (ly < 0 || ly >= a.rh) continue;
-                for (int txx = tx_lo; txx <= tx_hi; ++txx) {
-                    const int lx = x - (min(txx * tw_, a.W - 8) - a.rm);
-                    if (lx < 0 || lx >= a.rw) continue;
-                    const int* reg = scr + (((size_t)tyy * a.ntx + txx) * a.nplanes + p) * region;
-                    isum += reg[(z * a.rh + ly) * a.rw + lx];
+            for (int tyy = ylo[h], ly = ly0[h]; tyy <= yhi[h]; ++tyy) {
+                for (int txx = xlo, lx = lx0; txx <= xhi; ++txx) {
+                    const int* reg = scr + ((size_t)tyy * a.ntx + txx) * a.nplanes * region;
+                    if ((unsigned)lx < (unsigned)a.rw && (unsigned)ly < (unsigned)a.rh) isum += reg[ly * a.rw + lx];
+                    lx -= min((txx + 1) * tw_, a.W - 8) - min(txx * tw_, a.W - 8);
                 }
+                ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
             }
             num[h] = num[h] + (0.0f + (float)isum * fx_unit);
             den[h] = den[h] + dv;
