@@ -153,7 +153,7 @@ void dx_geometry(bm3d_plan* p, int max_lds) {
 }
 
 // Work-group size of the mixed filter kernel.
-constexpr int kMxFilterWaves = 4;
+constexpr int kMxFilterWaves = 8;
 
 // LDS bytes of the filter kernel for a tile of tbx x tby blocks and margin rm.
 size_t filter_lds(const bm3d_plan* p, int tbx, int tby, int rm, int* rw, int* rh) {

@@ -284,14 +284,25 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
                 const int r = q >> 1, col = 4 * (q & 1);
                 if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
                     int* qa = acc + (z * a.rh + ly + r) * a.rw + lx + col;
+                    if (E == 0.0f) {
+                        // no pre-rounding: the unit's power-of-two scale folds
+                        // into the weight exactly
+                        const float wsf = ws * fx_scale, wmf = wm * fx_scale;
 #pragma unroll
-                    for (int T = 0; T < 4; ++T)
+                        for (int T = 0; T < 4; ++T)
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            float val = fma_(ws, x[T][i], wm);
-                            val = (val + E) - E;
-                            atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(val * fx_scale));
-                        }
+                            for (int i = 0; i < 4; ++i)
+                                atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(fma_(wsf, x[T][i], wmf)));
+                    } else {
+#pragma unroll
+                        for (int T = 0; T < 4; ++T)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                float val = fma_(ws, x[T][i], wm);
+                                val = (val + E) - E;
+                                atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(val * fx_scale));
+                            }
+                    }
                 } else if (a.res) {
                     float* qr = res + (size_t)(2 * z) * a.plane_size + (size_t)(py + r) * a.stride + px + col;
 #pragma unroll
