@@ -27,9 +27,9 @@ first 5 frames, max-abs error and PSNR delta of both GPU paths against the
 FP64 CPU oracle running the same pipeline.
 
 cpu_baseline: the CPU oracle (a C restatement of the reference algorithm),
-frame-parallel over strips on host threads, on a bounded sample: one basic +
-one final BM3D window (radius 1) for 2 frames, each split into 8 full-width
-strips of 135 rows.
+frame-parallel over strips on host threads (one per core, at most 16), on a
+bounded sample: one basic + one final BM3D window (radius 1) for 4 frames,
+each split into 8 full-width strips of 135 rows (about 15 s of CPU work).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp32]
 """
@@ -76,7 +76,7 @@ def cpu_baseline(noisy, basic, threads):
     r = PARAMS["radius"]
     sig_b = [O.scale_sigma(PARAMS["sigma"], False)]
     sig_f = [O.scale_sigma(PARAMS["sigma"], True)]
-    frames = [10, 20]
+    frames = [5, 10, 15, 20]
     rows = 135
     jobs = []
     for n in frames:
@@ -198,7 +198,7 @@ def main():
                     help="after the run, fill and copy a buffer of this many MiB (a known-size "
                          "reference for HBM byte counters)")
     ap.add_argument("--no-compare", action="store_true", help="time only --precision")
-    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
