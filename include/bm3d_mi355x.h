@@ -169,6 +169,39 @@ int bm3d_plan_filter_kernel(const bm3d_plan *plan);
  * windows within its limits (BM3D_EUNSUPPORTED otherwise). */
 int bm3d_plan_select_kernels(bm3d_plan *plan, int matching, int filter);
 
+/* Launch shapes of a plan (no counterpart in the reference: the reference
+ * bakes one shape per GPU generation into its kernel, source/kernel.cu:
+ * 368-374, 823-828). Every shape gives the same results; only the speed
+ * differs. bm3d_plan_create picks defaults; bm3d_plan_autotune measures
+ * candidates on the caller's data. */
+typedef struct bm3d_tuning {
+    int match_rows;     /* rows of 16 reference blocks per matching work-group: 4, 8 or 16 */
+    int filter_tile_x;  /* reference blocks per filtering work-group tile: 1..32 each, */
+    int filter_tile_y;  /*   at most 256 in all                                         */
+    int filter_waves;   /* waves (64 lanes) per filtering work-group: 1..8 (the FP32
+                           filter uses at most 4) */
+    int batch_frames;   /* evaluations (output frames) per launch: 1..256 */
+    int chunks;         /* bm3d_execute_vagg: runs of evaluations per filtering tile and
+                           launch, 1..batch_frames (more runs: more work-groups, more
+                           regions summed with atomics) */
+} bm3d_tuning;
+
+int bm3d_plan_get_tuning(const bm3d_plan *plan, bm3d_tuning *out);
+/* Applies `t` (BM3D_EINVAL outside the ranges above, BM3D_EUNSUPPORTED when
+ * its work-group tiles do not fit the device's LDS). */
+int bm3d_plan_set_tuning(bm3d_plan *plan, const bm3d_tuning *t);
+
+/* Times candidate launch shapes on frames first_frame .. first_frame +
+ * count - 1 of src (and ref for a final-pass plan) with
+ * bm3d_execute_vagg (radius > 0) or bm3d_execute (radius 0), applies the
+ * fastest and records it in a process-wide cache keyed by the plan's
+ * parameters and plane size, so later plans with the same key start from it.
+ * When the environment variable BM3D_TUNE_CACHE names a file, the cache is
+ * read from and written to it (one line per key). Synchronises `stream`.
+ * The outputs written while tuning go to scratch, not to the caller. */
+int bm3d_plan_autotune(bm3d_plan *plan, const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count,
+                       void *stream);
+
 /* Kernel timing (no counterpart in the reference). While enabled, every
  * launch of the block-matching (kernel 0) and filtering (kernel 1) kernels
  * is bracketed by HIP events on its stream. bm3d_profile_read waits for the

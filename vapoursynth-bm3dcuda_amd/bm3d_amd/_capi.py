@@ -68,6 +68,14 @@ class Frames(ctypes.Structure):
     _fields_ = [("data", c_void_p * 3), ("frame_stride", c_ll * 3), ("stride", c_int * 3)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("match_rows", c_int), ("filter_tile_x", c_int), ("filter_tile_y", c_int),
+                ("filter_waves", c_int), ("batch_frames", c_int), ("chunks", c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BM3DError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(message)
@@ -101,6 +109,12 @@ def lib():
     L.bm3d_plan_filter_kernel.restype = c_int
     L.bm3d_plan_select_kernels.argtypes = [P, c_int, c_int]
     L.bm3d_plan_select_kernels.restype = c_int
+    L.bm3d_plan_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.bm3d_plan_get_tuning.restype = c_int
+    L.bm3d_plan_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.bm3d_plan_set_tuning.restype = c_int
+    L.bm3d_plan_autotune.argtypes = [P, ctypes.POINTER(Clip), ctypes.POINTER(Clip), c_int, c_int, P]
+    L.bm3d_plan_autotune.restype = c_int
     L.bm3d_execute.argtypes = [P, P, c_ll, c_ll, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
                                c_int, c_int, P]
     L.bm3d_execute.restype = c_int
@@ -160,7 +174,9 @@ def check(rc):
 EXPORTS = (
     "bm3d_plan_create", "bm3d_plan_destroy", "bm3d_stack_size", "bm3d_execute", "bm3d_execute_vagg",
     "bm3d_execute_window", "bm3d_aggregate", "bm3d_vaggregate", "bm3d_match",
-    "bm3d_num_blocks", "bm3d_plan_matching_kernel", "bm3d_plan_filter_kernel", "bm3d_plan_select_kernels", "bm3d_last_error", "bm3d_version", "bm3d_profile_enable",
+    "bm3d_num_blocks", "bm3d_plan_matching_kernel", "bm3d_plan_filter_kernel", "bm3d_plan_select_kernels",
+    "bm3d_plan_get_tuning", "bm3d_plan_set_tuning", "bm3d_plan_autotune", "bm3d_last_error", "bm3d_version",
+    "bm3d_profile_enable",
     "bm3d_profile_reset", "bm3d_profile_read", "bm3d_count_candidates",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
     "bm3d_filter_get_frames", "bm3d_filter_destroy",

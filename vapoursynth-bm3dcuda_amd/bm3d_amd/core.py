@@ -132,6 +132,36 @@ class Plan:
             raise ValueError("kernel must be 'fp32' or 'mixed'")
         C.check(C.lib().bm3d_plan_select_kernels(self._h, codes[matching], codes[filter]))
 
+    @property
+    def tuning(self):
+        """Launch shapes as a dict (match_rows, filter_tile_x, filter_tile_y,
+        filter_waves, batch_frames, chunks)."""
+        t = C.Tuning()
+        C.check(C.lib().bm3d_plan_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Changes some launch shapes; the others keep their values."""
+        t = C.Tuning()
+        C.check(C.lib().bm3d_plan_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        C.check(C.lib().bm3d_plan_set_tuning(self._h, ctypes.byref(t)))
+        return self.tuning
+
+    def autotune(self, src, ref=None, first=0, count=None, stream=None):
+        """Times candidate launch shapes on these frames and keeps the fastest."""
+        _check_clip(src, "clip")
+        n = src.shape[0]
+        count = n - first if count is None else count
+        s = clip_desc(src)
+        r = clip_desc(ref) if ref is not None else None
+        C.check(C.lib().bm3d_plan_autotune(self._h, ctypes.byref(s), ctypes.byref(r) if r else None,
+                                           int(first), int(count), _stream_ptr(stream)))
+        return self.tuning
+
     def count_candidates(self, src, ref=None, first=0, count=None, stream=None):
         """Exact number of in-plane block-matching candidates for the frames."""
         _check_clip(src, "clip")

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <limits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -45,8 +46,8 @@ int hip_fail(hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail(_e, #expr);      \
     } while (0)
 
-// Frames per launch: bounds the scratch (group lists and corner maps).
-constexpr int kMaxBatchFrames = 16;
+// Default evaluations per launch.
+constexpr int kDefaultBatchFrames = 32;
 
 // Optional per-kernel timing with HIP events (bm3d_profile_*).
 struct Profiler {
@@ -89,22 +90,29 @@ struct bm3d_plan {
     float extractor = 0.0f;
     int nplanes = 1, tw = 1;
     int nbx = 0, nby = 0, nblocks = 0;
-    int margin = 0, tby = 16, tile_w = 0, tile_h = 0, tile_wq = 0, tile_hp = 0;
-    size_t lds_bytes = 0;
-    long long plane_size = 0, stack_size = 0, corner_frame = 0;
+    int margin = 0;     // search reach: bm_range + radius * ps_range
+    long long plane_size = 0, stack_size = 0;
     int device = 0;
-    // filter work-group tiling and LDS accumulation region
-    int ftbx = 16, ftby = 8, fntx = 0, fnty = 0, frm = 0, frw = 0, frh = 0;
+    int max_lds = 0;
+    bm3d_tuning tune{};
+    // matching work-group tile: tby rows of 16 reference blocks plus the margin
+    int tby = 16, tile_w = 0, tile_h = 0, tile_wq = 0, tile_hp = 0;
+    size_t lds_bytes = 0;
+    // filter work-group tiling and LDS accumulation region (margin frm)
+    int ftbx = 8, ftby = 8, fntx = 0, fnty = 0, frm = 0, frw = 0, frh = 0;
     size_t filter_lds_bytes = 0;
-    long long scratch_frame = 0;  // floats of region scratch per frame
+    long long region_ints = 0;  // one region per tile and plane, all tiles: per slot or output frame
+    // device buffers, capacities in elements
     uint32_t* groups = nullptr;
+    size_t groups_cap = 0;
     float* corner = nullptr;
+    size_t corner_cap = 0;
     int* scratch = nullptr;
+    size_t scratch_cap = 0;
     // fixed-point unit of the accumulation regions: 2^fx_exp, scaled by the
     // input's range (measured into *range per call) unless extractor_exp is set
     int fx_exp = 0;
     float* range = nullptr;
-    int cap_frames = 0;
     // mixed-precision matching kernel (precision MIXED)
     bool use_dx = false, dx_ok = false;
     int tile_hp2 = 0;
@@ -122,82 +130,241 @@ int count_positions(int n, int bs) {
     return c;
 }
 
-// Tile geometry for a work-group of 16 x tby reference blocks.
-void tile_geometry(bm3d_plan* p, int tby) {
-    const int bs = p->d.block_step;
-    p->tby = tby;
-    p->tile_w = 15 * bs + 8 + 2 * p->margin;
-    p->tile_h = (tby - 1) * bs + 8 + 2 * p->margin;
-    p->tile_wq = (p->tile_w + bs - 1) / bs;
-    p->tile_hp = p->tile_h | 1;  // odd column pitch: lanes of a row hit distinct banks
-    p->lds_bytes = (size_t)bs * p->tile_wq * p->tile_hp * sizeof(float);
+// Device memory a plan's batch may take for its own buffers (group lists,
+// corner maps, regions); larger clips are processed in more launches.
+constexpr long long kScratchBudget = 8LL << 30;
+
+// Evaluations per launch: the tuning's batch, at most n, and at most what
+// the scratch budget allows at per_frame bytes each (after `fixed` bytes).
+int batch_limit(const bm3d_plan* p, int n, long long per_frame, long long fixed = 0) {
+    int b = std::min(n, p->tune.batch_frames);
+    const long long cap = (kScratchBudget - fixed) / std::max(per_frame, 1LL);
+    if (cap < b) b = (int)cap;
+    return std::max(b, 1);
 }
 
-int round_up(int v, int m) { return (v + m - 1) / m * m; }
+// Raises the dynamic-LDS limit of a kernel family (0 matching, 1 mixed
+// matching, 2 filtering, 3 mixed filtering) to at least `bytes`. The limit
+// is process-wide and only ever raised, so plans of different sizes coexist.
+hipError_t raise_lds_limit(int family, size_t bytes) {
+    static std::mutex mu;
+    static size_t cur[4] = {65536, 65536, 65536, 65536};
+    if (bytes <= 65536) return hipSuccess;
+    std::lock_guard<std::mutex> g(mu);
+    if (bytes <= cur[family]) return hipSuccess;
+    hipError_t e = hipSuccess;
+    switch (family) {
+        case 0: e = bm3d::set_match_lds_limit(bytes); break;
+        case 1: e = bm3d::set_match_dx_lds_limit(bytes); break;
+        case 2: e = bm3d::set_filter_lds_limit(bytes); break;
+        default: e = bm3d::set_filter_mx_lds_limit(bytes); break;
+    }
+    if (e == hipSuccess) cur[family] = bytes;
+    return e;
+}
+
+bm3d_tuning default_tuning(const bm3d_plan* p) {
+    bm3d_tuning t{};
+    t.match_rows = 16;
+    t.filter_tile_x = 8;
+    t.filter_tile_y = 8;
+    t.filter_waves = p->d.precision == BM3D_PRECISION_MIXED ? 8 : 4;
+    t.batch_frames = kDefaultBatchFrames;
+    t.chunks = 1;
+    return t;
+}
+
+const char* tuning_range_error(const bm3d_tuning& t) {
+    if (t.match_rows != 4 && t.match_rows != 8 && t.match_rows != 16) return "match_rows must be 4, 8 or 16";
+    if (t.filter_tile_x < 1 || t.filter_tile_x > 32 || t.filter_tile_y < 1 || t.filter_tile_y > 32 ||
+        t.filter_tile_x * t.filter_tile_y > 256)
+        return "filter tile must be 1..32 blocks per side and at most 256 blocks";
+    if (t.filter_waves < 1 || t.filter_waves > 8) return "filter_waves must be in range [1, 8]";
+    if (t.batch_frames < 1 || t.batch_frames > 256) return "batch_frames must be in range [1, 256]";
+    if (t.chunks < 1 || t.chunks > t.batch_frames) return "chunks must be in range [1, batch_frames]";
+    return nullptr;
+}
+
+// Process-wide cache of tuned launch shapes, keyed by the parameters and
+// plane size that set a plan's work (optionally backed by the file named in
+// BM3D_TUNE_CACHE: lines "<key> : <6 tuning fields>").
+struct TuneCache {
+    std::mutex mu;
+    bool loaded = false;
+    std::vector<std::pair<std::string, bm3d_tuning>> entries;
+};
+TuneCache& tune_cache() {
+    static TuneCache c;
+    return c;
+}
+
+std::string tune_key(const bm3d_plan* p) {
+    const bm3d_desc& d = p->d;
+    char buf[192];
+    std::snprintf(buf, sizeof buf, "w%d h%d s%d bs%d R%d r%d pn%d pr%d c%d f%d p%d", d.width, d.height, d.stride,
+                  d.block_step, d.bm_range, d.radius, d.ps_num, d.ps_range, d.chroma, d.final_, d.precision);
+    return buf;
+}
+
+void load_tune_file(TuneCache& c) {
+    if (c.loaded) return;
+    c.loaded = true;
+    const char* path = std::getenv("BM3D_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        char* colon = std::strchr(line, ':');
+        if (!colon) continue;
+        std::string key(line, colon - line);
+        while (!key.empty() && key.back() == ' ') key.pop_back();
+        bm3d_tuning t{};
+        if (std::sscanf(colon + 1, "%d %d %d %d %d %d", &t.match_rows, &t.filter_tile_x, &t.filter_tile_y,
+                        &t.filter_waves, &t.batch_frames, &t.chunks) == 6 && !tuning_range_error(t))
+            c.entries.emplace_back(key, t);
+    }
+    std::fclose(f);
+}
+
+bool cached_tuning(const bm3d_plan* p, bm3d_tuning* out) {
+    TuneCache& c = tune_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    load_tune_file(c);
+    const std::string key = tune_key(p);
+    for (auto it = c.entries.rbegin(); it != c.entries.rend(); ++it)
+        if (it->first == key) {
+            *out = it->second;
+            return true;
+        }
+    return false;
+}
+
+void store_tuning(const bm3d_plan* p, const bm3d_tuning& t) {
+    TuneCache& c = tune_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    load_tune_file(c);
+    const std::string key = tune_key(p);
+    bool found = false;
+    for (auto& e : c.entries)
+        if (e.first == key) {
+            e.second = t;
+            found = true;
+        }
+    if (!found) c.entries.emplace_back(key, t);
+    const char* path = std::getenv("BM3D_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return;
+    for (auto& e : c.entries)
+        std::fprintf(f, "%s : %d %d %d %d %d %d\n", e.first.c_str(), e.second.match_rows, e.second.filter_tile_x,
+                     e.second.filter_tile_y, e.second.filter_waves, e.second.batch_frames, e.second.chunks);
+    std::fclose(f);
+}
+
+// Matching tile for a work-group of 16 x rows reference blocks; false when
+// its LDS does not fit the device.
+bool match_geometry(bm3d_plan* p, int rows) {
+    const int bs = p->d.block_step;
+    const int tile_w = 15 * bs + 8 + 2 * p->margin;
+    const int tile_h = (rows - 1) * bs + 8 + 2 * p->margin;
+    const int tile_wq = (tile_w + bs - 1) / bs;
+    const int tile_hp = tile_h | 1;  // odd column pitch: lanes of a row hit distinct banks
+    const size_t bytes = (size_t)bs * tile_wq * tile_hp * sizeof(float);
+    if (bytes > (size_t)p->max_lds) return false;
+    p->tby = rows;
+    p->tile_w = tile_w;
+    p->tile_h = tile_h;
+    p->tile_wq = tile_wq;
+    p->tile_hp = tile_hp;
+    p->lds_bytes = bytes;
+    return true;
+}
 
 // Mixed-precision matching kernel (bm3d_match_dx.hip): the FP32 kernel's
 // tile, stored as two planes of FP16 pairs (hp2 words per column, odd so
 // that the 16 lanes of a row hit distinct banks). Scan indices are packed
 // in 12 bits.
-void dx_geometry(bm3d_plan* p, int max_lds) {
+void dx_geometry(bm3d_plan* p) {
     const bm3d_desc& d = p->d;
-    p->use_dx = false;
+    p->dx_ok = false;
     if (d.precision != BM3D_PRECISION_MIXED) return;
     const int nWs = 2 * d.bm_range + 1, nWt = 2 * d.ps_range + 1;
     if (nWs * nWs > 4096 || (d.radius > 0 && d.ps_num * nWt * nWt > 4096)) return;
     p->tile_hp2 = ((p->tile_h + 1) / 2 + 1) | 1;
     const size_t bytes = (size_t)2 * d.block_step * p->tile_wq * p->tile_hp2 * 4;
-    if (bytes > (size_t)max_lds) return;
+    if (bytes + 8192 > (size_t)p->max_lds) return;  // + the kernel's static list buffer
     p->dx_lds_bytes = bytes;
-    p->use_dx = p->dx_ok = true;
+    p->dx_ok = true;
 }
 
-// Work-group size of the mixed filter kernel.
-constexpr int kMxFilterWaves = 8;
-
-// LDS bytes of the filter kernel for a tile of tbx x tby blocks and margin rm.
-size_t filter_lds(const bm3d_plan* p, int tbx, int tby, int rm, int* rw, int* rh) {
+// Filter tiling: tbx x tby reference blocks per work-group, accumulation
+// margin rm; false when the LDS regions (2r + 1 of them) do not fit.
+bool filter_geometry(bm3d_plan* p, int tbx, int tby, int rm) {
     const int bs = p->d.block_step;
-    *rw = (tbx - 1) * bs + 8 + 2 * rm;
-    *rh = (tby - 1) * bs + 8 + 2 * rm;
-    return (size_t)p->tw * *rw * *rh * sizeof(float);
+    const int rw = (tbx - 1) * bs + 8 + 2 * rm;
+    const int rh = (tby - 1) * bs + 8 + 2 * rm;
+    const size_t bytes = (size_t)p->tw * rw * rh * sizeof(int);
+    if (bytes + 1024 > (size_t)p->max_lds) return false;  // + the kernels' static slot table
+    p->ftbx = tbx; p->ftby = tby; p->frm = rm; p->frw = rw; p->frh = rh;
+    p->filter_lds_bytes = bytes;
+    p->fntx = (p->nbx + tbx - 1) / tbx;
+    p->fnty = (p->nby + tby - 1) / tby;
+    p->region_ints = (long long)p->fntx * p->fnty * p->nplanes * rw * rh;
+    return true;
 }
 
-// Filter tiling: the largest accumulation margin up to the full search reach
-// (bm_range + radius * ps_range) whose region fits the LDS budget; members
-// beyond a reduced margin fall back to global atomics.
-void filter_geometry(bm3d_plan* p, size_t budget) {
-    const int full = p->margin;
-    const int tiles[][2] = {{8, 8}, {8, 4}};
-    for (auto& t : tiles) {
-        for (int rm = full; rm >= 0; --rm) {
-            int rw, rh;
-            size_t b = filter_lds(p, t[0], t[1], rm, &rw, &rh);
-            if (b <= budget && (rm == full || (t[0] == 8 && t[1] == 4))) {
-                p->ftbx = t[0]; p->ftby = t[1]; p->frm = rm; p->frw = rw; p->frh = rh;
-                p->filter_lds_bytes = b;
-                p->fntx = (p->nbx + t[0] - 1) / t[0];
-                p->fnty = (p->nby + t[1] - 1) / t[1];
-                p->scratch_frame = (long long)p->fntx * p->fnty * p->nplanes * p->tw * rw * rh;
-                return;
-            }
-        }
+// Applies a tuning's tiles: the requested ones at the full search reach
+// when they fit, else the largest default tile that fits, else a 4 x 4 tile
+// with a reduced margin (members beyond it go to global atomics, and
+// bm3d_execute_vagg is unavailable).
+int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
+    if (!match_geometry(p, t.match_rows) && !match_geometry(p, 4))
+        return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
+    dx_geometry(p);
+    if (!p->dx_ok) p->use_dx = false;
+    bool ok = filter_geometry(p, t.filter_tile_x, t.filter_tile_y, p->margin);
+    const int fallback[][2] = {{8, 8}, {8, 4}, {4, 4}};
+    for (auto& f : fallback)
+        if (!ok) ok = filter_geometry(p, f[0], f[1], p->margin);
+    for (int rm = p->margin - 1; !ok && rm >= 0; --rm) ok = filter_geometry(p, 4, 4, rm);
+    if (!ok) return fail(BM3D_EUNSUPPORTED, "BM3D: no filter tiling fits the device's LDS");
+    hipError_t e = raise_lds_limit(0, p->lds_bytes);
+    if (e == hipSuccess && p->dx_ok) e = raise_lds_limit(1, p->dx_lds_bytes);
+    if (e == hipSuccess) e = raise_lds_limit(2, p->filter_lds_bytes);
+    if (e == hipSuccess) e = raise_lds_limit(3, p->filter_lds_bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    p->tune = t;
+    p->tune.match_rows = p->tby;
+    p->tune.filter_tile_x = p->ftbx;
+    p->tune.filter_tile_y = p->ftby;
+    return BM3D_OK;
+}
+
+// Grows the plan's device buffers to hold at least the given numbers of
+// group-list entries, corner-map floats and region ints.
+int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch) {
+    if (groups > p->groups_cap) {
+        if (p->groups) (void)hipFree(p->groups);
+        p->groups = nullptr;
+        p->groups_cap = 0;
+        BM3D_HIP(hipMalloc(&p->groups, groups * sizeof(uint32_t)));
+        p->groups_cap = groups;
     }
-}
-
-int ensure_scratch(bm3d_plan* p, int frames) {
-    if (frames <= p->cap_frames) return BM3D_OK;
-    if (p->groups) (void)hipFree(p->groups);
-    if (p->corner) (void)hipFree(p->corner);
-    if (p->scratch) (void)hipFree(p->scratch);
-    p->groups = nullptr;
-    p->corner = nullptr;
-    p->scratch = nullptr;
-    p->cap_frames = 0;
-    BM3D_HIP(hipMalloc(&p->groups, (size_t)frames * p->nblocks * 8 * sizeof(uint32_t)));
-    BM3D_HIP(hipMalloc(&p->corner, (size_t)frames * p->corner_frame * sizeof(float)));
-    BM3D_HIP(hipMalloc(&p->scratch, (size_t)frames * p->scratch_frame * sizeof(int)));
-    p->cap_frames = frames;
+    if (corner > p->corner_cap) {
+        if (p->corner) (void)hipFree(p->corner);
+        p->corner = nullptr;
+        p->corner_cap = 0;
+        BM3D_HIP(hipMalloc(&p->corner, corner * sizeof(float)));
+        p->corner_cap = corner;
+    }
+    if (scratch > p->scratch_cap) {
+        if (p->scratch) (void)hipFree(p->scratch);
+        p->scratch = nullptr;
+        p->scratch_cap = 0;
+        BM3D_HIP(hipMalloc(&p->scratch, scratch * sizeof(int)));
+        p->scratch_cap = scratch;
+    }
     return BM3D_OK;
 }
 
@@ -243,40 +410,15 @@ hipError_t measure_range(bm3d_plan* p, const bm3d_clip* src, int lo, int hi, hip
                                p->nplanes, p->d.width, p->d.height, p->d.stride, hi - lo + 1, p->range, stream);
 }
 
-// Block matching and filtering of frames n0 .. n0 + nf - 1: group lists,
-// the tile accumulation regions (scratch) and the corner weight maps. With
-// ring > 0, frame n's regions and corner maps go to index n % ring of the
-// plan's buffers (which hold `ring` frames), else to index n - n0.
-hipError_t evaluate(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int n0, int nf, float* res,
-                    long long res_frame_stride, long long res_plane_stride, int ring, hipStream_t stream) {
+// Filter arguments common to both accumulation modes.
+bm3d::FilterArgs filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
     const bm3d_desc& d = p->d;
-    hipError_t e = hipSuccess;
-    if (ring > 0) {
-        // corner maps of the ring slots of n0 .. n0 + nf - 1 (may wrap)
-        int f = 0;
-        while (f < nf && e == hipSuccess) {
-            const int slot = (n0 + f) % ring;
-            const int run = std::min(nf - f, ring - slot);
-            e = bm3d::launch_zero(p->corner + (size_t)slot * p->corner_frame, (size_t)run * p->corner_frame, 0, 1,
-                                  stream);
-            f += run;
-        }
-    } else {
-        e = bm3d::launch_zero(p->corner, (size_t)nf * p->corner_frame, 0, 1, stream);
-    }
-    if (e != hipSuccess) return e;
-    {
-        ProfScope ps(0, stream);
-        e = launch_matching(p, p->groups, nullptr, src, ref, n0, nf, stream);
-    }
-    if (e != hipSuccess) return e;
     bm3d::FilterArgs fa{};
     fa.src = src->data;
     fa.ref = d.final_ ? ref->data : nullptr;
     fa.frame_stride = src->frame_stride;
     fa.plane_stride = src->plane_stride;
     fa.num_frames = src->num_frames;
-    fa.first_frame = n0;
     fa.W = d.width; fa.H = d.height; fa.stride = d.stride;
     fa.plane_size = p->plane_size;
     fa.radius = d.radius;
@@ -284,31 +426,54 @@ hipError_t evaluate(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, in
     for (int i = 0; i < 3; ++i) fa.sigma[i] = p->sigma[i];
     fa.extractor = p->extractor;
     fa.groups = p->groups;
-    fa.res = res;
-    fa.res_frame_stride = res_frame_stride;
-    fa.res_plane_stride = res_plane_stride;
     fa.corner = p->corner;
-    fa.corner_frame_stride = p->corner_frame;
     fa.bs = d.block_step; fa.nbx = p->nbx; fa.nby = p->nby;
     fa.tbx = p->ftbx; fa.tby = p->ftby; fa.ntx = p->fntx; fa.nty = p->fnty;
     fa.rm = p->frm; fa.rw = p->frw; fa.rh = p->frh;
     fa.scratch = p->scratch;
     fa.fx_range = fx_range(p);
     fa.fx_exp = p->fx_exp;
-    fa.ring = ring;
-    {
-        ProfScope ps(1, stream);
-        if (p->use_mx_filter) {
-            fa.mx_ops = p->mx_ops;
-            e = bm3d::launch_filter_mx(fa, d.final_ != 0, d.chroma != 0, nf, kMxFilterWaves, p->filter_lds_bytes,
-                                       stream);
-        } else {
-            e = bm3d::launch_filter(fa, d.final_ != 0, d.chroma != 0, nf, p->filter_lds_bytes, stream);
-        }
-    }
-    return e;
+    fa.mx_ops = p->mx_ops;
+    return fa;
 }
 
+hipError_t launch_filtering(const bm3d_plan* p, const bm3d::FilterArgs& fa, int grid_z, hipStream_t stream) {
+    ProfScope ps(1, stream);
+    const bool fin = p->d.final_ != 0, chroma = p->d.chroma != 0;
+    if (p->use_mx_filter)
+        return bm3d::launch_filter_mx(fa, fin, chroma, grid_z, std::min(p->tune.filter_waves, bm3d::kMaxMxFilterWaves),
+                                      p->filter_lds_bytes, stream);
+    return bm3d::launch_filter(fa, fin, chroma, grid_z, std::min(p->tune.filter_waves, bm3d::kMaxFilterWaves),
+                               p->filter_lds_bytes, stream);
+}
+
+// Slot-mode evaluation of frames n0 .. n0 + nf - 1 (stacks path): group
+// lists, slot regions and corner maps per evaluation.
+hipError_t evaluate_slots(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int n0, int nf, float* res,
+                          long long res_frame_stride, long long res_plane_stride, hipStream_t stream) {
+    const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
+    hipError_t e = bm3d::launch_zero(p->corner, (size_t)nf * corner_eval, 0, 1, stream);
+    if (e != hipSuccess) return e;
+    {
+        ProfScope ps(0, stream);
+        e = launch_matching(p, p->groups, nullptr, src, ref, n0, nf, stream);
+    }
+    if (e != hipSuccess) return e;
+    bm3d::FilterArgs fa = filter_args(p, src, ref);
+    fa.mode = 0;
+    fa.first_frame = n0;
+    fa.res = res;
+    fa.res_frame_stride = res_frame_stride;
+    fa.res_plane_stride = res_plane_stride;
+    fa.corner_frame_stride = corner_eval;
+    return launch_filtering(p, fa, nf, stream);
+}
+
+// Evaluations that can add to output frame t in target mode:
+// contributors_lo(t) .. contributors_hi(t) (a slot of evaluation m covers
+// frame clamp(m - r + z), z in [0, 2r]).
+int contributors_lo(int t, int r) { return std::max(t - r, 0); }
+int contributors_hi(int t, int r, int N) { return std::min(t + r, N - 1); }
 
 }  // namespace
 
@@ -364,52 +529,20 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     p->margin = d.bm_range + d.radius * d.ps_range;
     p->plane_size = (long long)d.height * d.stride;
     p->stack_size = (long long)p->nplanes * p->tw * 2 * p->plane_size;
-    p->corner_frame = (long long)p->nplanes * p->tw * p->plane_size;
 
     hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&p->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
     if (e != hipSuccess) {
         delete p;
-        return hip_fail(e, "hipGetDevice");
+        return hip_fail(e, "device set-up");
     }
-    int max_lds = 0;
-    e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
-    if (e != hipSuccess) {
+    bm3d_tuning t = default_tuning(p);
+    (void)cached_tuning(p, &t);
+    if (int rc = apply_tuning(p, t)) {
         delete p;
-        return hip_fail(e, "hipDeviceGetAttribute");
+        return rc;
     }
-    tile_geometry(p, 16);
-    if (p->lds_bytes > 65536) tile_geometry(p, 4);
-    if (p->lds_bytes > (size_t)max_lds) {
-        delete p;
-        return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
-    }
-    if (p->lds_bytes > 65536) {
-        e = bm3d::set_match_lds_limit(p->lds_bytes);
-        if (e != hipSuccess) {
-            delete p;
-            return hip_fail(e, "hipFuncSetAttribute");
-        }
-    }
-    filter_geometry(p, std::min<size_t>((size_t)max_lds, 52 * 1024));
-    if (p->filter_lds_bytes == 0 || p->filter_lds_bytes > (size_t)max_lds) {
-        delete p;
-        return fail(BM3D_EUNSUPPORTED, "BM3D: no filter tiling fits the device's LDS");
-    }
-    if (p->filter_lds_bytes > 65536) {
-        e = bm3d::set_filter_lds_limit(p->filter_lds_bytes);
-        if (e != hipSuccess) {
-            delete p;
-            return hip_fail(e, "hipFuncSetAttribute");
-        }
-    }
-    dx_geometry(p, max_lds);
-    if (p->use_dx && p->dx_lds_bytes > 65536) {
-        e = bm3d::set_match_dx_lds_limit(p->dx_lds_bytes);
-        if (e != hipSuccess) {
-            delete p;
-            return hip_fail(e, "hipFuncSetAttribute");
-        }
-    }
+    p->use_dx = p->dx_ok;
     e = hipMalloc(&p->range, sizeof(float));
     if (e != hipSuccess) {
         bm3d_plan_destroy(p);
@@ -421,7 +554,6 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         const size_t bytes = ops.size() * sizeof(uint32_t);
         e = hipMalloc(&p->mx_ops, bytes);
         if (e == hipSuccess) e = hipMemcpy(p->mx_ops, ops.data(), bytes, hipMemcpyHostToDevice);
-        if (e == hipSuccess && p->filter_lds_bytes > 65536) e = bm3d::set_filter_mx_lds_limit(p->filter_lds_bytes);
         if (e != hipSuccess) {
             bm3d_plan_destroy(p);
             return hip_fail(e, "mixed filter set-up");
@@ -465,8 +597,12 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
     BM3D_HIP(hipGetDevice(&dev));
     if (dev != p->device) return fail(BM3D_EINVAL, "BM3D: plan used on a different device than it was created on");
 
-    const int batch = std::min(count, kMaxBatchFrames);
-    if (int rc = ensure_scratch(p, batch)) return rc;
+    const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
+    const long long scratch_eval = (long long)p->tw * p->region_ints;
+    const int batch = batch_limit(p, count, (corner_eval + scratch_eval) * 4 + (long long)p->nblocks * 32);
+    if (int rc = ensure_buffers(p, (size_t)batch * p->nblocks * 8, (size_t)batch * corner_eval,
+                                (size_t)batch * scratch_eval))
+        return rc;
 
     const bm3d_desc& d = p->d;
     BM3D_HIP(measure_range(p, src, std::max(first_frame - d.radius, 0),
@@ -488,7 +624,7 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
                                                (size_t)res_plane_stride, p->nplanes, stream));
             }
         }
-        BM3D_HIP(evaluate(p, src, ref, n0, nf, res, res_frame_stride, res_plane_stride, 0, stream));
+        BM3D_HIP(evaluate_slots(p, src, ref, n0, nf, res, res_frame_stride, res_plane_stride, stream));
 
         bm3d::FinalizeArgs fz{};
         fz.scratch = p->scratch;
@@ -510,7 +646,7 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
 
         bm3d::DenArgs da{};
         da.corner = p->corner;
-        da.corner_frame_stride = p->corner_frame;
+        da.corner_frame_stride = corner_eval;
         da.res = res;
         da.res_frame_stride = res_frame_stride;
         da.res_plane_stride = res_plane_stride;
@@ -582,6 +718,112 @@ int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
     return BM3D_OK;
 }
 
+int bm3d_plan_get_tuning(const bm3d_plan* p, bm3d_tuning* out) {
+    g_last_error.clear();
+    if (!p || !out) return fail(BM3D_EINVAL, "BM3D: null argument");
+    *out = p->tune;
+    return BM3D_OK;
+}
+
+int bm3d_plan_set_tuning(bm3d_plan* p, const bm3d_tuning* t) {
+    g_last_error.clear();
+    if (!p || !t) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (const char* why = tuning_range_error(*t)) return fail(BM3D_EINVAL, std::string("BM3D: tuning: ") + why);
+    const bm3d_tuning old = p->tune;
+    if (int rc = apply_tuning(p, *t)) {
+        const std::string msg = g_last_error;
+        (void)apply_tuning(p, old);
+        return fail(rc, msg);
+    }
+    return BM3D_OK;
+}
+
+int bm3d_plan_autotune(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
+                       void* stream_) {
+    g_last_error.clear();
+    if (!p || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (src->num_frames <= 0 || count <= 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    const bm3d_desc& d = p->d;
+    const bool fused = d.radius > 0 && p->frm >= p->margin;
+    const size_t out_floats = fused ? (size_t)count * p->nplanes * d.height * d.width : (size_t)count * p->stack_size;
+    float* out = nullptr;
+    BM3D_HIP(hipMalloc(&out, out_floats * sizeof(float)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    int rc = he == hipSuccess ? BM3D_OK : hip_fail(he, "hipEventCreate");
+    auto run = [&]() -> int {
+        if (fused) return bm3d_execute_vagg(p, out, 0, 0, 0, src, ref, first_frame, count, stream_);
+        return bm3d_execute(p, out, 0, 0, src, ref, first_frame, count, stream_);
+    };
+    // milliseconds of one run with tuning t (warm), or a negative value when
+    // t does not apply
+    auto time_one = [&](const bm3d_tuning& t) -> double {
+        if (tuning_range_error(t) || apply_tuning(p, t) != BM3D_OK) return -1.0;
+        if (run() != BM3D_OK) return -1.0;
+        float best = -1.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            if (hipEventRecord(e0, stream) != hipSuccess || run() != BM3D_OK ||
+                hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                return -1.0;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+            best = best < 0.0f ? ms : std::min(best, ms);
+        }
+        return best;
+    };
+    bm3d_tuning best = p->tune;
+    double best_ms = rc == BM3D_OK ? time_one(best) : -1.0;
+    if (best_ms < 0.0 && rc == BM3D_OK)
+        rc = fail(BM3D_EHIP, "BM3D: autotune: the plan's launch shapes failed: " + g_last_error);
+    auto consider = [&](const bm3d_tuning& t) {
+        const double ms = time_one(t);
+        if (ms >= 0.0 && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    if (rc == BM3D_OK) {
+        // coordinate descent: matching rows, then filter tile and waves,
+        // then batch size and evaluation runs
+        for (int rows : {16, 8, 4}) {
+            bm3d_tuning t = best;
+            t.match_rows = rows;
+            consider(t);
+        }
+        const int tiles[][3] = {{8, 8, 4}, {8, 8, 8}, {16, 8, 4}, {16, 8, 8}, {8, 16, 8}, {16, 16, 8},
+                                {8, 4, 4}, {8, 4, 2}, {4, 4, 2}};
+        const bm3d_tuning base = best;
+        for (auto& f : tiles) {
+            bm3d_tuning t = base;
+            t.filter_tile_x = f[0];
+            t.filter_tile_y = f[1];
+            t.filter_waves = f[2];
+            consider(t);
+        }
+        const bm3d_tuning base2 = best;
+        for (int b : {8, 16, 32, 64, 128}) {
+            for (int c : {1, 2, 4}) {
+                bm3d_tuning t = base2;
+                t.batch_frames = b;
+                t.chunks = std::min(c, b);
+                if (c > 1 && !fused) continue;
+                consider(t);
+            }
+        }
+        (void)apply_tuning(p, best);
+        store_tuning(p, p->tune);
+        g_last_error.clear();
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    return rc;
+}
+
 int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
                           unsigned long long* ncand, void* stream) {
     g_last_error.clear();
@@ -644,16 +886,28 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     // evaluations needed: frames clamp(n - r .. n + r) of every output n
     const int mlo = std::max(first_frame - r, 0);
     const int mhi = std::min(first_frame + count - 1 + r, N - 1);
-    const int batch = std::min(kMaxBatchFrames, mhi - mlo + 1);
-    // resident evaluations: an output's 2r + 1 plus the batch in flight
-    const int ring = std::min(batch + 2 * r, mhi - mlo + 1);
-    if (int rc = ensure_scratch(p, ring)) return rc;
+    // output frames the evaluations add to
+    const int tlo_all = std::max(mlo - r, 0), thi_all = std::min(mhi + r, N - 1);
+    const long long corner_target = (long long)p->nplanes * p->plane_size;
+    const long long target_bytes = (corner_target + p->region_ints) * 4;
+    const int batch = batch_limit(p, mhi - mlo + 1, target_bytes + (long long)p->nblocks * 32, 2 * r * target_bytes);
+    // resident output frames: those a launch adds to (its evaluations +- r)
+    const int ring = std::min(batch + 2 * r, thi_all - tlo_all + 1);
+    if (int rc = ensure_buffers(p, (size_t)batch * p->nblocks * 8, (size_t)ring * corner_target,
+                                (size_t)ring * p->region_ints))
+        return rc;
 
-    bm3d::VRegArgs va{};
+    bm3d::FilterArgs fa = filter_args(p, src, ref);
+    fa.mode = 1;
+    fa.ring = ring;
+    fa.target_stride = p->region_ints;
+    fa.corner_frame_stride = corner_target;
+
+    bm3d::VTargetArgs va{};
     va.scratch = p->scratch;
-    va.scratch_frame = p->scratch_frame;
+    va.target_stride = p->region_ints;
     va.corner = p->corner;
-    va.corner_frame = p->corner_frame;
+    va.corner_frame = corner_target;
     va.ring = ring;
     va.fx_range = fx_range(p);
     va.fx_exp = p->fx_exp;
@@ -666,23 +920,53 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     va.plane_mask = 0;
     for (int i = 0; i < p->nplanes; ++i)
         if (!(d.chroma && p->sigma[i] < std::numeric_limits<float>::epsilon())) va.plane_mask |= 1u << i;
-    va.radius = r;
-    va.num_frames = N;
     va.bs = d.block_step; va.tbx = p->ftbx; va.tby = p->ftby; va.ntx = p->fntx; va.nty = p->fnty;
     va.rm = p->frm; va.rw = p->frw; va.rh = p->frh;
 
     BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), stream));
     int next_out = first_frame;
+    int touched_hi = tlo_all - 1;  // highest output frame a previous launch added to
     for (int b0 = mlo; b0 <= mhi;) {
         const int b1 = std::min(b0 + batch, mhi + 1);
-        BM3D_HIP(evaluate(p, src, ref, b0, b1 - b0, nullptr, 0, 0, ring, stream));
-        // outputs whose evaluations are all resident
+        const int nf = b1 - b0;
+        const int chunk = (nf + p->tune.chunks - 1) / std::max(1, p->tune.chunks);
+        const int nchunks = (nf + chunk - 1) / chunk;
+        // output frames first added to by this launch start from zero: the
+        // corner maps always (float atomics), the regions when they are
+        // summed atomically (contributions from more than one run)
+        const int tnew_lo = std::max(std::max(b0 - r, 0), touched_hi + 1);
+        const int tnew_hi = std::min(b1 - 1 + r, N - 1);
+        for (int t = tnew_lo; t <= tnew_hi;) {
+            const int slot = t % ring;
+            const int run = std::min(tnew_hi - t + 1, ring - slot);
+            BM3D_HIP(bm3d::launch_zero(p->corner + (size_t)slot * corner_target, (size_t)run * corner_target, 0, 1,
+                                       stream));
+            t += run;
+        }
+        for (int t = tnew_lo; t <= tnew_hi; ++t) {
+            const int clo = contributors_lo(t, r), chi = contributors_hi(t, r, N);
+            const int k = (clo - b0) / chunk;
+            const bool whole = clo >= b0 && chi < b1 && chi < b0 + (k + 1) * chunk;
+            if (!whole)
+                BM3D_HIP(bm3d::launch_zero(reinterpret_cast<float*>(p->scratch) + (size_t)(t % ring) * p->region_ints,
+                                           (size_t)p->region_ints, 0, 1, stream));
+        }
+        touched_hi = std::max(touched_hi, tnew_hi);
+        {
+            ProfScope ps(0, stream);
+            BM3D_HIP(launch_matching(p, p->groups, nullptr, src, ref, b0, nf, stream));
+        }
+        fa.eval_lo = b0;
+        fa.eval_hi = b1;
+        fa.chunk = chunk;
+        BM3D_HIP(launch_filtering(p, fa, nchunks, stream));
+        // outputs whose evaluations are all done
         const int out_hi = (b1 - 1 == N - 1) ? first_frame + count - 1
                                               : std::min(first_frame + count - 1, b1 - 1 - r);
         if (out_hi >= next_out) {
             va.first_frame = next_out;
             va.dst = dst + (size_t)(next_out - first_frame) * dst_frame_stride;
-            BM3D_HIP(bm3d::launch_vagg_regions(va, out_hi - next_out + 1, stream));
+            BM3D_HIP(bm3d::launch_vagg_targets(va, out_hi - next_out + 1, stream));
             next_out = out_hi + 1;
         }
         b0 = b1;

@@ -96,5 +96,100 @@ template <typename T> __device__ __forceinline__ T lane_xor4(T v) { return dpp_m
 // Value of lane k of the quad (quad_perm [k, k, k, k]).
 template <int kLane, typename T> __device__ __forceinline__ T quad_bcast(T v) { return dpp_mov<kLane * 0x55>(v); }
 
+// Number of i in [0, 2r] for which VAggregate's output frame
+// n = clamp(m - r + z, 0, N - 1) reads slot z of the evaluation of frame m:
+// it reads slot clamp(2r - i, n - N + 1 + r, n + r) of the evaluation of
+// frame clamp(n - r + i, 0, N - 1) (source/source.cpp:834-836). 1 inside
+// the clip; 0, 2 or more for evaluations within r of its ends.
+__device__ __forceinline__ int vagg_multiplicity(int m, int z, int r, int N) {
+    const int n = clampi(m - r + z, 0, N - 1);
+    int c = 0;
+    for (int i = 0; i <= 2 * r; ++i) {
+        const int mi = clampi(n - r + i, 0, N - 1);
+        const int si = clampi(2 * r - i, n - N + 1 + r, n + r);
+        c += (mi == m && si == z) ? 1 : 0;
+    }
+    return c;
+}
+
+// Where the members of one evaluation go, per temporal slot z: the offset
+// of the LDS accumulation region, the number of times each member is added,
+// and the offset of the weight map (floats from FilterArgs::corner).
+struct SlotTable {
+    int region[kMaxSlots];
+    int mult[kMaxSlots];
+    long long corner[kMaxSlots];
+};
+
+// Walks the evaluations of a filter work-group's tile (grid (ntx, nty, z);
+// FilterArgs::mode) plane by plane: zeroes the LDS regions `acc`, fills the
+// slot table of each evaluation, runs body(p, m, gf) -- every group of the
+// tile for evaluation frame m, group lists at index gf -- and writes the
+// regions to scratch: per evaluation (mode 0), or per output frame once the
+// evaluations that add to it are done (mode 1). LDS: tw regions of rh x rw.
+template <typename Body>
+__device__ __forceinline__ void filter_walk(const FilterArgs& a, int* acc, SlotTable& st, int nplanes, bool chroma,
+                                            Body&& body) {
+    const int r = a.radius, tw = 2 * r + 1, N = a.num_frames;
+    const int regsize = a.rh * a.rw;
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    int e0, e1;
+    if (a.mode == 0) {
+        e0 = a.first_frame + (int)blockIdx.z;
+        e1 = e0 + 1;
+    } else {
+        e0 = a.eval_lo + (int)blockIdx.z * a.chunk;
+        e1 = min(e0 + a.chunk, a.eval_hi);
+        if (e0 >= e1) return;  // uniform over the work-group
+    }
+    for (int p = 0; p < nplanes; ++p) {
+        if (chroma && a.sigma[p] < kFltEpsilon) continue;  // source/kernel.cu:657-668
+        for (int i = threadIdx.x; i < tw * regsize; i += blockDim.x) acc[i] = 0;
+        int flushed = max(e0 - r, 0) - 1;  // mode 1: last output frame written
+        for (int m = e0; m < e1; ++m) {
+            __syncthreads();  // the previous evaluation's table and regions are done with
+            if ((int)threadIdx.x < tw) {
+                const int z = threadIdx.x;
+                if (a.mode == 0) {
+                    st.region[z] = z * regsize;
+                    st.mult[z] = 1;
+                    st.corner[z] = (long long)blockIdx.z * a.corner_frame_stride + (long long)(p * tw + z) * a.plane_size;
+                } else {
+                    const int t = clampi(m - r + z, 0, N - 1);
+                    st.region[z] = (t % tw) * regsize;
+                    st.mult[z] = vagg_multiplicity(m, z, r, N);
+                    st.corner[z] = (long long)(t % a.ring) * a.corner_frame_stride + (long long)p * a.plane_size;
+                }
+            }
+            __syncthreads();
+            body(p, m, a.mode == 0 ? (int)blockIdx.z : m - a.eval_lo);
+            __syncthreads();
+            if (a.mode == 1) {
+                // frames whose every contributing evaluation (within r) is done
+                const int hi = (m == N - 1 || m == e1 - 1) ? min(m + r, N - 1) : m - r;
+                for (int t = flushed + 1; t <= hi; ++t) {
+                    const bool whole = max(t - r, 0) >= e0 && min(t + r, N - 1) <= e1 - 1;
+                    int* src = acc + (t % tw) * regsize;
+                    int* dst = a.scratch + (size_t)(t % a.ring) * a.target_stride +
+                               (((size_t)ty * a.ntx + tx) * nplanes + p) * regsize;
+                    for (int i = threadIdx.x; i < regsize; i += blockDim.x) {
+                        const int v = src[i];
+                        if (whole) dst[i] = v;
+                        else if (v != 0) atomicAdd(dst + i, v);
+                        src[i] = 0;
+                    }
+                }
+                flushed = max(flushed, hi);
+            }
+        }
+        if (a.mode == 0) {
+            __syncthreads();
+            int* out = a.scratch + ((((size_t)blockIdx.z * a.nty + ty) * a.ntx + tx) * nplanes + p) * tw * regsize;
+            for (int i = threadIdx.x; i < tw * regsize; i += blockDim.x) out[i] = acc[i];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace dev
 }  // namespace bm3d

@@ -5,7 +5,7 @@
 // the MFMA pipe (FP32 accumulation); shrinkage, weights and aggregation stay
 // in FP32 and use the FP32 kernel's accumulation scheme (int32 fixed-point
 // LDS regions per work-group tile, corner weight map), so the consumers of
-// the regions (bm3d_finalize, bm3d_vagg_regions) are shared.
+// the regions (bm3d_finalize, bm3d_vagg_targets) are shared.
 //
 // A wave filters two groups (16 member blocks b) per step. With D the 8-point
 // DCT-II (4 x orthonormal) and K = D (x) D acting on a block's 64 pixels:
@@ -167,39 +167,32 @@ __device__ __forceinline__ float row_sum16(float s) {
     return s;
 }
 
-// Grid (ntx, nty, frames), blockDim.x = 64 x waves. Same tile, region and
-// scratch layout as bm3d_filter (bm3d_kernels.hip).
+// Grid (ntx, nty, grid_z), blockDim.x = 64 x waves. Same tile, region and
+// scratch layout, and the same accumulation modes, as bm3d_filter
+// (bm3d_kernels.hip, filter_walk).
 template <bool kFinal, bool kChroma>
 __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
     extern __shared__ float lds[];
-    const int tw = 2 * a.radius + 1;
-    const int region = tw * a.rh * a.rw;
+    __shared__ SlotTable st;
     int* acc = reinterpret_cast<int*>(lds);
     const int nwaves = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int q = lane >> 4, c = lane & 15;
-    const int tx = blockIdx.x, ty = blockIdx.y, f = blockIdx.z;
-    const int n = a.first_frame + f;
+    const int tx = blockIdx.x, ty = blockIdx.y;
     const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
     const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
-    const int nplanes = kChroma ? 3 : 1;
-    const int slot = a.ring > 0 ? n % a.ring : f;
     const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
 
     MxOps o;
     o.load(reinterpret_cast<const uint4*>(a.mx_ops), lane);
 
-    for (int p = 0; p < nplanes; ++p) {
+    filter_walk(a, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
-        if (kChroma && sigma < kFltEpsilon) continue;  // source/kernel.cu:657-668
-        for (int i = threadIdx.x; i < region; i += blockDim.x) acc[i] = 0;
-        __syncthreads();
         const float* src = a.src + (size_t)p * a.plane_stride;
-        float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
-        float* corner = a.corner + (size_t)slot * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
+        float* res = a.res ? a.res + (size_t)gf * a.res_frame_stride + (size_t)p * a.res_plane_stride : nullptr;
 
         for (int base = 0; base < ngroups; base += nwaves * 2) {
             const int li = base + wave * 2 + (c >> 3);
@@ -207,7 +200,7 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
             const int by = ty * a.tby + li / a.tbx;
             const bool active = li < ngroups && bx < a.nbx && by < a.nby;
             const int blk = active ? by * a.nbx + bx : 0;
-            const uint32_t pk = a.groups[((size_t)f * a.nblocks + blk) * 8 + (c & 7)];
+            const uint32_t pk = a.groups[((size_t)gf * a.nblocks + blk) * 8 + (c & 7)];
             // positions come from block matching and are in range; the clamps
             // only keep a corrupted list from addressing outside the planes
             const int z = min(pos_z(pk), 2 * a.radius);
@@ -274,7 +267,8 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
             f4 x[4];
             inverse_mx(x, o, zc);
 
-            if (active) {
+            const int mult = st.mult[z];
+            if (active && mult > 0) {
                 // aggregation with pre-rounding (source/kernel.cu:717-756);
                 // the transform gain 4096 is divided out with the weight
                 const float w = (c >> 3) ? w1 : w0;
@@ -283,7 +277,8 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
                 const int lx = px - rx0, ly = py - ry0;
                 const int r = q >> 1, col = 4 * (q & 1);
                 if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
-                    int* qa = acc + (z * a.rh + ly + r) * a.rw + lx + col;
+                    int* qa = acc + st.region[z] + (ly + r) * a.rw + lx + col;
+                    int iv[4][4];
                     if (E == 0.0f) {
                         // no pre-rounding: the unit's power-of-two scale folds
                         // into the weight exactly
@@ -291,8 +286,7 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
 #pragma unroll
                         for (int T = 0; T < 4; ++T)
 #pragma unroll
-                            for (int i = 0; i < 4; ++i)
-                                atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(fma_(wsf, x[T][i], wmf)));
+                            for (int i = 0; i < 4; ++i) iv[T][i] = (int)__builtin_rintf(fma_(wsf, x[T][i], wmf));
                     } else {
 #pragma unroll
                         for (int T = 0; T < 4; ++T)
@@ -300,10 +294,17 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
                             for (int i = 0; i < 4; ++i) {
                                 float val = fma_(ws, x[T][i], wm);
                                 val = (val + E) - E;
-                                atomicAdd(qa + 2 * T * a.rw + i, (int)__builtin_rintf(val * fx_scale));
+                                iv[T][i] = (int)__builtin_rintf(val * fx_scale);
                             }
                     }
-                } else if (a.res) {
+                    // a slot read `mult` times by VAggregate is added as often
+                    for (int k = 0; k < mult; ++k) {
+#pragma unroll
+                        for (int T = 0; T < 4; ++T)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) atomicAdd(qa + 2 * T * a.rw + i, iv[T][i]);
+                    }
+                } else if (res) {
                     float* qr = res + (size_t)(2 * z) * a.plane_size + (size_t)(py + r) * a.stride + px + col;
 #pragma unroll
                     for (int T = 0; T < 4; ++T)
@@ -316,15 +317,12 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
                 }
                 if (q == 0) {
                     const float wr = (w + E) - E;
-                    unsafeAtomicAdd(corner + (size_t)z * a.plane_size + (size_t)py * a.stride + px, wr);
+                    float* cq = a.corner + st.corner[z] + (size_t)py * a.stride + px;
+                    for (int k = 0; k < mult; ++k) unsafeAtomicAdd(cq, wr);
                 }
             }
         }
-        __syncthreads();
-        int* out = a.scratch + ((((size_t)slot * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
-        for (int i = threadIdx.x; i < region; i += blockDim.x) out[i] = acc[i];
-        __syncthreads();
-    }
+    });
 }
 
 }  // namespace
@@ -360,9 +358,9 @@ void build_filter_mx_ops(uint32_t* out) {
     }
 }
 
-hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int frames, int waves, size_t lds_bytes,
+hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                             hipStream_t s) {
-    dim3 grid(a.ntx, a.nty, frames), block(64 * waves);
+    dim3 grid(a.ntx, a.nty, grid_z), block(64 * waves);
     if (final_) {
         if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<true, true>), grid, block, lds_bytes, s, a);
         else hipLaunchKernelGGL((bm3d_filter_mx<true, false>), grid, block, lds_bytes, s, a);

@@ -17,8 +17,8 @@ constexpr int kMaxWidth = 1 << 14;
 constexpr int kMaxHeight = 1 << 13;
 constexpr int kMaxRadius = 15;
 
-// Waves per filter work-group (8 groups per wave).
-constexpr int kFilterWaves = 4;
+// Largest temporal width 2r + 1 (the filters' per-slot tables).
+constexpr int kMaxSlots = 2 * kMaxRadius + 1;
 
 struct MatchArgs {
     const float* clip;       // plane used for matching, frame 0
@@ -54,11 +54,30 @@ struct FilterArgs {
     int bs, nbx, nby;
     int tbx, tby, ntx, nty;  // work-group tile of reference blocks, tile grid
     int rm, rw, rh;          // accumulation region: margin, width, height
-    int* scratch;            // [frames][nty][ntx][planes][2r+1][rh][rw], fixed point
+    int* scratch;            // fixed-point regions (layout by mode, below)
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
-    int ring;                // > 0: frame n's scratch and corner maps at index n % ring
     const void* mx_ops;      // mixed filter: per-lane transform operands (build_filter_mx_ops)
+    // Accumulation mode.
+    //  0 (slots): one evaluation per work-group (frame first_frame + z of the
+    //    grid); region of slot z of evaluation f at scratch
+    //    [f][nty][ntx][planes][2r+1][rh][rw], weights at corner
+    //    [f][planes][2r+1][H][stride] (corner_frame_stride floats per f).
+    //  1 (targets): a work-group walks evaluations eval_lo + chunk * z ..
+    //    (at most `chunk`, below eval_hi) of its tile, and sums slot z of
+    //    evaluation m, `multiplicity` times, into output frame
+    //    t = clamp(m - r + z, 0, num_frames - 1) -- the VAggregate sum of
+    //    source/source.cpp:816-846 regrouped by the frame each slot covers.
+    //    Region of frame t at scratch + (t % ring) * target_stride +
+    //    [nty][ntx][planes][rh][rw]; weights at corner + (t % ring) *
+    //    corner_frame_stride + [planes][H][stride]. A region whose
+    //    contributing evaluations all lie in the work-group's chunk is
+    //    stored, any other is added atomically (the caller zeroes it first).
+    //    Group lists of evaluation m at index m - eval_lo.
+    int mode;
+    int eval_lo, eval_hi, chunk;
+    int ring;
+    long long target_stride;
 };
 
 struct FinalizeArgs {
@@ -97,14 +116,15 @@ struct DivideArgs {
     int nplanes;
 };
 
-// VAggregate straight from the filter's accumulation regions and corner maps
-// (bm3d_vagg_regions): the stacks are never written.
-struct VRegArgs {
-    const int* scratch;      // ring of evaluations: [ring][nty][ntx][planes][2r+1][rh][rw]
-    long long scratch_frame; // ints per evaluation
-    const float* corner;     // [ring][planes][2r+1][H][stride]
-    long long corner_frame;  // floats per evaluation
-    int ring;                // evaluation of frame m at index m % ring
+// VAggregate from target-mode accumulation (bm3d_vagg_targets): output
+// frame n = the regions' integer sum of frame n (times the unit) over the
+// 8 x 8 box sum of its corner weight map. The stacks are never written.
+struct VTargetArgs {
+    const int* scratch;      // [ring][nty][ntx][planes][rh][rw]
+    long long target_stride; // ints per frame
+    const float* corner;     // [ring][planes][H][stride]
+    long long corner_frame;  // floats per frame
+    int ring;                // frame n at index n % ring
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
     float* dst;              // output frames [first_frame, first_frame + count)
@@ -114,7 +134,7 @@ struct VRegArgs {
     long long plane_size;
     int nplanes;             // planes of the evaluation (1, or 3 with chroma)
     unsigned plane_mask;     // planes to produce (bit p)
-    int radius, num_frames, first_frame;
+    int first_frame;
     int bs, tbx, tby, ntx, nty, rm, rw, rh;
 };
 
@@ -139,14 +159,18 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
 hipError_t set_match_dx_lds_limit(size_t bytes);
 // Keys kept per temporal frame (at least 2 ps_num).
 int dx_temporal_keys(int ps_num);
-hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
+// Grid (ntx, nty, grid_z): grid_z = evaluations (mode 0) or chunks (mode 1);
+// `waves` waves per work-group (1..kMaxFilterWaves).
+constexpr int kMaxFilterWaves = 4;
+constexpr int kMaxMxFilterWaves = 8;
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                          hipStream_t s);
 hipError_t set_filter_lds_limit(size_t bytes);
 // Mixed-precision filter (bm3d_filter_mx.hip): same arguments, tiles and
-// outputs as launch_filter; `waves` waves per work-group (1..8).
+// outputs as launch_filter; `waves` waves per work-group (1..kMaxMxFilterWaves).
 constexpr int kMxOpVectors = 17;  // uint4 per lane of FilterArgs::mx_ops
 void build_filter_mx_ops(uint32_t* out);  // kMxOpVectors * 64 * 4 words, host
-hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int frames, int waves, size_t lds_bytes,
+hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                             hipStream_t s);
 hipError_t set_filter_mx_lds_limit(size_t bytes);
 hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s);
@@ -159,6 +183,6 @@ hipError_t launch_absmax(const float* clip, long long frame_stride, long long pl
 hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s);
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s);
 hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s);
-hipError_t launch_vagg_regions(const VRegArgs& a, int count, hipStream_t s);
+hipError_t launch_vagg_targets(const VTargetArgs& a, int count, hipStream_t s);
 
 }  // namespace bm3d

@@ -12,10 +12,13 @@
 //                  candidate) and keeps its top-8 in registers.
 //   bm3d_filter    8 lanes per group, 8 groups per wave: gather, 3-D DCT,
 //                  hard threshold or Wiener shrinkage, inverse, and
-//                  aggregation with float atomics (pre-rounded when
-//                  extractor_exp is set). Per-member weights are added once
-//                  at the member's corner; bm3d_den_box turns the corner map
-//                  into the per-pixel weight rows with an 8x8 box sum.
+//                  aggregation (pre-rounded when extractor_exp is set) into
+//                  int32 fixed-point LDS regions, per slot of an evaluation
+//                  or per output frame of a run of evaluations. Per-member
+//                  weights are added once at the member's corner.
+//   bm3d_finalize / bm3d_den_box   stacks from the slot regions and corner
+//                  maps (8x8 box sum of the corners).
+//   bm3d_vagg_targets   VAggregate straight from the output-frame regions.
 //   bm3d_zero / bm3d_divide / bm3d_vaggregate   elementwise passes.
 //
 // Every float operation is written out (the build uses -ffp-contract=off),
@@ -265,47 +268,39 @@ __device__ __forceinline__ void inverse3d(float (&v)[8][8], int k) {
 }
 
 // Collaborative filtering (source/kernel.cu:649-756). A work-group of
-// nwaves waves owns a tile of tbx x tby reference blocks and filters their
-// groups 8 per wave (lane k of a group holds column k; member transposes go
-// through DPP lane exchanges, not LDS). Weighted estimates are accumulated
-// as int32 fixed point (units of a.fx_unit) with integer LDS atomics -- LDS
-// float atomics are several times slower -- into a per-slot region covering
-// the tile plus `rm` pixels (every member of a tile's groups lies inside it
-// when rm >= bm_range + radius * ps_range); the region is then written to
-// scratch with plain stores and bm3d_finalize sums overlapping regions.
-// With extractor_exp the pre-rounded values are whole multiples of the unit,
-// so the integer sums are exact. Members outside the region go to global
-// float atomics on the stack. Member weights go to the corner map (one
-// atomic per member). Grid: (ntx, nty, frames).
+// blockDim.x / 64 waves owns a tile of tbx x tby reference blocks and filters
+// their groups 8 per wave (lane k of a group holds column k; member
+// transposes go through DPP lane exchanges, not LDS). Weighted estimates are
+// accumulated as int32 fixed point (units of 2^fx_exponent) with integer LDS
+// atomics -- LDS float atomics are several times slower -- into regions
+// covering the tile plus `rm` pixels (every member of a tile's groups lies
+// inside them when rm >= bm_range + radius * ps_range), per temporal slot of
+// one evaluation or per output frame over a run of evaluations
+// (FilterArgs::mode, filter_walk). With extractor_exp the pre-rounded values
+// are whole multiples of the unit, so the integer sums are exact. Members
+// outside the regions (mode 0 only) go to global float atomics on the
+// stack. Member weights go to a corner map (one atomic per member).
 template <bool kFinal, bool kChroma>
 __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
     extern __shared__ float lds[];
-    const int tw = 2 * a.radius + 1;
-    const int region = tw * a.rh * a.rw;
+    __shared__ SlotTable st;
     int* acc = reinterpret_cast<int*>(lds);
     const int nwaves = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 3;
     const int k = lane & 7;
-    const int tx = blockIdx.x, ty = blockIdx.y, f = blockIdx.z;
-    const int n = a.first_frame + f;
+    const int tx = blockIdx.x, ty = blockIdx.y;
     const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
     const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
-    const int nplanes = kChroma ? 3 : 1;
     const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
 
-    for (int p = 0; p < nplanes; ++p) {
+    filter_walk(a, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
-        if (kChroma && sigma < kFltEpsilon) continue;  // source/kernel.cu:657-668
-        for (int i = threadIdx.x; i < region; i += blockDim.x) acc[i] = 0;
-        __syncthreads();
         const float* src = a.src + (size_t)p * a.plane_stride;
-        float* res = a.res + (size_t)f * a.res_frame_stride + (size_t)p * a.res_plane_stride;
-        const int slot = a.ring > 0 ? n % a.ring : f;
-        float* corner = a.corner + (size_t)slot * a.corner_frame_stride + (size_t)p * tw * a.plane_size;
+        float* res = a.res ? a.res + (size_t)gf * a.res_frame_stride + (size_t)p * a.res_plane_stride : nullptr;
 
         for (int base = 0; base < ngroups; base += nwaves * 8) {
             const int li = base + wave * 8 + g;
@@ -316,7 +311,7 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
 
             uint32_t pk[8];
             {
-                const uint4* gp = reinterpret_cast<const uint4*>(a.groups + ((size_t)f * a.nblocks + blk) * 8);
+                const uint4* gp = reinterpret_cast<const uint4*>(a.groups + ((size_t)gf * a.nblocks + blk) * 8);
                 const uint4 q0 = gp[0], q1 = gp[1];
                 pk[0] = q0.x; pk[1] = q0.y; pk[2] = q0.z; pk[3] = q0.w;
                 pk[4] = q1.x; pk[5] = q1.y; pk[6] = q1.z; pk[7] = q1.w;
@@ -403,16 +398,24 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int z = pos_z(pk[i]), px = pos_x(pk[i]), py = pos_y(pk[i]);
+                    const int mult = st.mult[z];
+                    if (mult == 0) continue;
                     const int lx = px - rx0, ly = py - ry0;
                     if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
-                        int* q = acc + (z * a.rh + ly) * a.rw + lx + k;
+                        int* q = acc + st.region[z] + ly * a.rw + lx + k;
+                        int iv[8];
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             float val = w * v[i][j];
                             val = (val + E) - E;
-                            atomicAdd(q + j * a.rw, (int)__builtin_rintf(val * fx_scale));
+                            iv[j] = (int)__builtin_rintf(val * fx_scale);
                         }
-                    } else if (a.res) {
+                        // a slot read `mult` times by VAggregate is added as often
+                        for (int c = 0; c < mult; ++c) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) atomicAdd(q + j * a.rw, iv[j]);
+                        }
+                    } else if (res) {
                         float* q = res + (size_t)(2 * z) * a.plane_size + (size_t)py * a.stride + px + k;
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
@@ -421,16 +424,14 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
                             unsafeAtomicAdd(q + (size_t)j * a.stride, val);
                         }
                     }
-                    if (k == 0)
-                        unsafeAtomicAdd(corner + (size_t)z * a.plane_size + (size_t)py * a.stride + px, wr);
+                    if (k == 0) {
+                        float* cq = a.corner + st.corner[z] + (size_t)py * a.stride + px;
+                        for (int c = 0; c < mult; ++c) unsafeAtomicAdd(cq, wr);
+                    }
                 }
             }
         }
-        __syncthreads();
-        int* out = a.scratch + ((((size_t)slot * a.nty + ty) * a.ntx + tx) * nplanes + p) * region;
-        for (int i = threadIdx.x; i < region; i += blockDim.x) out[i] = acc[i];
-        __syncthreads();
-    }
+    });
 }
 
 }  // namespace
@@ -620,15 +621,15 @@ __device__ __forceinline__ void covering_tiles(int v, int span, float inv_span, 
     while (lo <= hi && min(lo * span, limit) - rm + rlen <= v) ++lo;
 }
 
-// VAggregate of output frames first_frame .. first_frame + count - 1 straight
-// from the evaluations' tile regions and corner maps (the stacks are never
-// formed): for slot i = 0..2r (source/source.cpp:834-836), frame
-// m = clamp(n - r + i) holds the evaluation and z = clamp(2r - i, ...) the
-// slot; num = the regions' integer sum times the unit (as bm3d_finalize),
-// den = the corner map's 8 x 8 box sum (as bm3d_den_box, same order), both
-// added over i in order from 0, then num / den. Work-group: 64 x 8 pixels,
-// grid (ceil(W/64), ceil(H/8), count * nplanes).
-__global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
+// VAggregate of output frames first_frame .. first_frame + count - 1 from
+// target-mode accumulation (FilterArgs::mode 1): the filter has already
+// summed, per output frame n, every slot VAggregate reads for n
+// (source/source.cpp:816-846) into integer regions and one corner weight
+// map. num = the covering tiles' region values summed as integers, times
+// the unit; den = the corner map's 8 x 8 box sum (rows, then columns, as
+// bm3d_den_box); out = num / den. Work-group: 64 x 8 pixels, grid
+// (ceil(W/64), ceil(H/8), count * nplanes).
+__global__ __launch_bounds__(256) void bm3d_vagg_targets(VTargetArgs a) {
     __shared__ float cm[15][72];      // corner-map window: rows y0-7.., columns x0-7..
     __shared__ float rowsum[15][64];
     const int tx = threadIdx.x & 63;
@@ -639,68 +640,53 @@ __global__ __launch_bounds__(256) void bm3d_vagg_regions(VRegArgs a) {
     const int fo = blockIdx.z / a.nplanes, p = blockIdx.z % a.nplanes;
     if (!((a.plane_mask >> p) & 1u)) return;  // uniform per work-group
     const int n = a.first_frame + fo;
-    const int r = a.radius;
-    const int tw = 2 * r + 1;
-    const int region = tw * a.rh * a.rw;
+    const int slot = n % a.ring;
     const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
     const float fx_unit = ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
-    // covering tiles and region coordinates of this thread's two pixels
+    const float* cmap = a.corner + (size_t)slot * a.corner_frame + (size_t)p * a.plane_size;
+    for (int k = threadIdx.x; k < 15 * 71; k += 256) {
+        const int rr = k / 71, cc = k - rr * 71;
+        const int yy = y0 - 7 + rr, xx = x0 - 7 + cc;
+        cm[rr][cc] = (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) ? cmap[(size_t)yy * a.stride + xx] : 0.0f;
+    }
+    // covering tiles and region coordinates of this thread's pixels
     int xlo, xhi;
     covering_tiles(min(x, a.W - 1), tw_, 1.0f / (float)tw_, a.W - 8, a.rm, a.rw, a.ntx, xlo, xhi);
     const int lx0 = x - (min(xlo * tw_, a.W - 8) - a.rm);
-    int ylo[2], yhi[2], ly0[2];
+    const int region = a.rh * a.rw;
+    const int* scr = a.scratch + (size_t)slot * a.target_stride + (size_t)p * region;
+    __syncthreads();
+    for (int rr = ty; rr < 15; rr += 4) {
+        float s = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) s = s + cm[rr][tx + 7 - b];
+        rowsum[rr][tx] = s;
+    }
+    __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int y = min(y0 + ty + 4 * h, a.H - 1);
-        covering_tiles(y, th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo[h], yhi[h]);
-        ly0[h] = y - (min(ylo[h] * th, a.H - 8) - a.rm);
-    }
-    float num[2] = {0.0f, 0.0f}, den[2] = {0.0f, 0.0f};
-    for (int i = 0; i < tw; ++i) {
-        const int m = clampi(n - r + i, 0, a.num_frames - 1);
-        const int z = clampi(2 * r - i, n - a.num_frames + 1 + r, n + r);
-        const int slot = m % a.ring;
-        const float* cmap = a.corner + (size_t)slot * a.corner_frame + (size_t)(p * tw + z) * a.plane_size;
-        __syncthreads();
-        for (int k = threadIdx.x; k < 15 * 71; k += 256) {
-            const int rr = k / 71, cc = k - rr * 71;
-            const int yy = y0 - 7 + rr, xx = x0 - 7 + cc;
-            cm[rr][cc] = (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) ? cmap[(size_t)yy * a.stride + xx] : 0.0f;
-        }
-        __syncthreads();
-        for (int rr = ty; rr < 15; rr += 4) {
-            float s = 0.0f;
+        const int jj = ty + 4 * h;
+        const int y = y0 + jj;
+        int ylo, yhi;
+        covering_tiles(min(y, a.H - 1), th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo, yhi);
+        float dv = 0.0f;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) s = s + cm[rr][tx + 7 - b];
-            rowsum[rr][tx] = s;
-        }
-        __syncthreads();
-        const int* scr = a.scratch + (size_t)slot * a.scratch_frame + (size_t)p * region + (size_t)z * a.rh * a.rw;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int jj = ty + 4 * h;
-            float dv = 0.0f;
-#pragma unroll
-            for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
-            long long isum = 0;
-            for (int tyy = ylo[h], ly = ly0[h]; tyy <= yhi[h]; ++tyy) {
-                for (int txx = xlo, lx = lx0; txx <= xhi; ++txx) {
-                    const int* reg = scr + ((size_t)tyy * a.ntx + txx) * a.nplanes * region;
-                    if ((unsigned)lx < (unsigned)a.rw && (unsigned)ly < (unsigned)a.rh) isum += reg[ly * a.rw + lx];
-                    lx -= min((txx + 1) * tw_, a.W - 8) - min(txx * tw_, a.W - 8);
-                }
-                ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
+        for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
+        long long isum = 0;
+        int ly = min(y, a.H - 1) - (min(ylo * th, a.H - 8) - a.rm);
+        for (int tyy = ylo; tyy <= yhi; ++tyy) {
+            int lx = lx0;
+            for (int txx = xlo; txx <= xhi; ++txx) {
+                const int* reg = scr + ((size_t)tyy * a.ntx + txx) * a.nplanes * region;
+                if ((unsigned)lx < (unsigned)a.rw && (unsigned)ly < (unsigned)a.rh) isum += reg[ly * a.rw + lx];
+                lx -= min((txx + 1) * tw_, a.W - 8) - min(txx * tw_, a.W - 8);
             }
-            num[h] = num[h] + (0.0f + (float)isum * fx_unit);
-            den[h] = den[h] + dv;
+            ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
         }
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int y = y0 + ty + 4 * h;
+        const float num = 0.0f + (float)isum * fx_unit;
         if (y < a.H && x < a.W)
             a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] =
-                num[h] / den[h];
+                num / dv;
     }
 }
 
@@ -724,10 +710,10 @@ hipError_t set_match_lds_limit(size_t bytes) {
     return hipSuccess;
 }
 
-hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int frames, size_t lds_bytes,
+hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                          hipStream_t s) {
-    dim3 grid(a.ntx, a.nty, frames);
-    dim3 block(64 * kFilterWaves);
+    dim3 grid(a.ntx, a.nty, grid_z);
+    dim3 block(64 * waves);
     if (final_) {
         if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true>), grid, block, lds_bytes, s, a);
         else hipLaunchKernelGGL((bm3d_filter<true, false>), grid, block, lds_bytes, s, a);
@@ -784,9 +770,9 @@ hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_vagg_regions(const VRegArgs& a, int count, hipStream_t s) {
+hipError_t launch_vagg_targets(const VTargetArgs& a, int count, hipStream_t s) {
     dim3 grid((a.W + 63) / 64, (a.H + 7) / 8, count * a.nplanes);
-    hipLaunchKernelGGL(bm3d_vagg_regions, grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(bm3d_vagg_targets, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

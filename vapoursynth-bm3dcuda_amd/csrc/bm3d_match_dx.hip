@@ -10,7 +10,7 @@
 // Layout as in the FP32 kernel: one lane per reference block, 16 x TBY
 // blocks per work-group, the work-group's search windows staged in LDS one
 // slot at a time, each lane sliding an 8-column ring across a window. The
-// tile is stored as FP16 pairs of vertically adjacent pixels minus a tile
+// tile is stored as FP16 pairs of vertically adjacent pixels minus an
 // offset mu (two copies, pairs starting on even and on odd rows), so a
 // candidate column of 8 rows is 4 packed registers. Per candidate:
 //
@@ -24,8 +24,9 @@
 // are rescored exactly and ordered by (distance, scan index), which is the
 // order the reference's strict sorted insertion in scan order produces; the
 // list equals the FP32 path's unless an exact top-k candidate ranks below 2k
-// approximately. A slot whose pixels are not finite or span more than +-8
-// around mu is searched exhaustively in FP32, exactly as the FP32 kernel.
+// approximately. A work-group tile whose pixels are not finite or reach
+// beyond +-8 around mu (the centre pixel of the slot's frame) is searched
+// exhaustively in FP32, exactly as the FP32 kernel.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -386,12 +387,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     };
     const float* plane_r = slot_plane(r);
 
-    // Stages slot z as FP16 pairs offset by the tile's centre pixel mu;
-    // returns false (the slot is then scanned exactly from memory) when a
-    // pixel is not finite or further than kDxMaxSpan from mu.
+    // Stages slot z as FP16 pairs offset by the centre pixel mu of the
+    // slot's frame (the same for every work-group, so the pre-scores do not
+    // depend on the launch shape); returns false (the slot is then scanned
+    // exactly from memory) when a pixel of the tile is not finite or further
+    // than kDxMaxSpan from mu.
     auto stage = [&](const float* img, float& mu) -> bool {
-        const int cxm = clampi(pg.tx0 + pg.tw / 2, 0, a.W - 1), cym = clampi(pg.ty0 + pg.th / 2, 0, a.H - 1);
-        mu = img[(size_t)cym * a.stride + cxm];
+        mu = img[(size_t)(a.H / 2) * a.stride + a.W / 2];
         if (!(__builtin_fabsf(mu) <= 1e30f)) mu = 0.0f;
         return stage_pairs(lds, pg, img, a.W, a.H, a.stride, mu);
     };
