@@ -10,7 +10,10 @@ block_step=4, bm_range=9, ps_num=2, ps_range=4, radius=1:
 (each pass runs the BM3D evaluations over a ring of frames and the
 VAggregate average straight from their accumulation regions; the per-frame
 stacks of the two-stage form are never written -- the outputs are identical).
-The clip is resident in HBM before the timed region; value = frames/s of
+Launch shapes (tiles, waves, frames per launch) come from the plans'
+autotuner, run on the same clip before the timed region (--no-autotune:
+the plans' defaults). The clip is resident in HBM before the timed region;
+value = frames/s of
 the whole two-pass job over K steps, for the mixed-precision path
 (--precision, default mixed: FP16 block-matching pre-scoring with exact FP32
 rescoring of the best candidates). The plain FP32 path is timed the same way
@@ -31,7 +34,13 @@ frame-parallel over strips on host threads (one per core, at most 16), on a
 bounded sample: one basic + one final BM3D window (radius 1) for 4 frames,
 each split into 8 full-width strips of 135 rows (about 15 s of CPU work).
 
+batch workload (--workload batch): 64 seeded GRAYS clips of 8 frames, each
+drawing its size (360p..2160p), radius (0..2) and sigma (5..25); two-pass
+BM3Dv2 through one captured batch graph, against the same clips run one by
+one through FP32 plans. Prints its own JSON line (aggregate Mpixel/s).
+
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp32]
+                [--workload vbm3d|batch]
 """
 from __future__ import annotations
 
@@ -120,15 +129,24 @@ class Pipeline:
         self.basic = torch.empty_like(src)
         self.final = torch.empty_like(src)
 
+    def autotune(self):
+        """Launch shapes for both passes from the plans' autotuner (untimed)."""
+        self.pb.execute_vagg(self.src, out=self.basic)
+        tb = self.pb.autotune(self.src)
+        tf = self.pf.autotune(self.src, self.basic)
+        return {"basic": tb, "final": tf}
+
     def step(self, stream=None):
         # BM3Dv2 = BM3D + VAggregate, fused: the stacks are not materialised
         self.pb.execute_vagg(self.src, out=self.basic, stream=stream)
         self.pf.execute_vagg(self.src, self.basic, out=self.final, stream=stream)
 
 
-def time_path(core, capi, src, precision, steps, warmup):
+def time_path(core, capi, src, precision, steps, warmup, autotune):
     import torch
     pipe = Pipeline(core, src, precision)
+    tuning = pipe.autotune() if autotune else {"basic": pipe.pb.tuning, "final": pipe.pf.tuning}
+    log(f"{precision}: launch shapes {tuning}")
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         pipe.step(stream)
@@ -156,7 +174,7 @@ def time_path(core, capi, src, precision, steps, warmup):
         "fps": src.shape[0] * steps / dt, "ms_per_step": dt / steps * 1e3,
         "match_ms_step": match_ms / steps, "filter_ms_step": filt_ms / steps,
         "avg_launch_ms": avg_ms, "flops_launch": flops_launch, "achieved_tflops": achieved,
-        "ncand": ncand, "pipe": pipe,
+        "ncand": ncand, "pipe": pipe, "tuning": tuning,
     }
 
 
@@ -185,6 +203,107 @@ def oracle_crop_errors(core, noisy, clean):
     return out
 
 
+BATCH_SIZES = [(360, 640), (720, 1280), (1080, 1920), (2160, 3840)]
+BATCH_PARAMS = dict(block_step=4, bm_range=9, ps_num=2, ps_range=4)
+
+
+def batch_workload(n_clips=64, frames=8):
+    """The 64-clip serving mix: clip k draws its resolution from 360p..2160p,
+    radius from {0, 1, 2} and sigma from U[5, 25] with seed k; content and
+    integer motion as the main workload, noise sd = sigma / 255."""
+    import numpy as np
+    from bm3d_amd import synth
+    clips = []
+    for k in range(n_clips):
+        rng = np.random.default_rng(k)
+        h, w = BATCH_SIZES[int(rng.integers(0, 4))]
+        radius = int(rng.integers(0, 3))
+        sigma = float(rng.uniform(5.0, 25.0))
+        _, noisy = synth.make_clip(h, w, frames, 1, noise_sd=sigma, seed=k, motion=True)
+        clips.append(dict(noisy=noisy, radius=radius, sigma=sigma))
+    return clips
+
+
+def run_batch(args):
+    """Aggregate Mpixel/s of the 64-clip mix: one batched graph (mixed
+    precision, and FP32) vs per-clip FP32 plan launches, two passes each."""
+    import torch
+    from bm3d_amd import core
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    clips = batch_workload()
+    log(f"generated {len(clips)} clips in {time.perf_counter() - t0:.1f}s")
+    srcs = [torch.from_numpy(c["noisy"]).cuda() for c in clips]
+    outs = [torch.empty_like(s) for s in srcs]
+    basics = [torch.empty_like(s) for s in srcs]
+    px = sum(s.numel() for s in srcs)  # output pixels of one run (GRAYS)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    # per-clip FP32 launches: the two passes of each clip through its plans
+    plans = []
+    for c, s in zip(clips, srcs):
+        _, _, h, w = s.shape
+        kw = dict(BATCH_PARAMS, radius=c["radius"], precision="fp32")
+        plans.append((core.Plan(w, h, c["sigma"], **kw), core.Plan(w, h, c["sigma"], final=True, **kw)))
+
+    def per_clip():
+        for (pb, pf), s, o, b in zip(plans, srcs, outs, basics):
+            pb.execute_vagg(s, out=b)
+            pf.execute_vagg(s, b, out=o)
+
+    t_clip = timed(per_clip, args.steps)
+    log(f"per-clip FP32: {t_clip * 1e3:.1f} ms per run")
+    ref_basic = [b.clone() for b in basics]
+    del plans
+    res = {"per_clip_fp32_mpixel_s": round(px / t_clip / 1e6, 1)}
+    for prec in ("fp32", "mixed"):
+        jobs = [dict(BATCH_PARAMS, src=s, out=o, basic=b, passes=2, radius=c["radius"], sigma=[c["sigma"]],
+                     precision=prec) for c, s, o, b in zip(clips, srcs, outs, basics)]
+        batch = core.Batch(jobs)
+        if not args.no_autotune:
+            batch.autotune()
+        t_b = timed(batch.execute, args.steps)
+        info = batch.info
+        log(f"batch {prec}: {t_b * 1e3:.1f} ms per run, {info}")
+        res[f"batch_{prec}_mpixel_s"] = round(px / t_b / 1e6, 1)
+        res[f"batch_{prec}_ms"] = round(t_b * 1e3, 3)
+        res[f"batch_{prec}_graph"] = dict(info, launch_shapes=batch.tuning)
+        if prec == "fp32":
+            # The batched FP32 graph reproduces the per-clip FP32 basic
+            # estimates up to the order of the float weight atomics (the
+            # final pass matches on the basic estimate, so an ulp there can
+            # flip a near-tie group; with extractor_exp both are bit-exact,
+            # see tests/test_gpu_batch.py).
+            res["batch_fp32_basic_max_abs_vs_per_clip"] = max(float((b - r).abs().max())
+                                                              for b, r in zip(basics, ref_basic))
+        del batch
+    out = {
+        "metric": "aggregate Mpixel/s, 64-clip batch of two-pass BM3Dv2 (360p..2160p, radius 0..2)",
+        "value": res["batch_mixed_mpixel_s"],
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "higher_is_better": True,
+        "dtype": "f16+f32",
+        "data": "synthetic (64 seeded clips, 8 frames each, sizes/radius/sigma drawn per clip)",
+        "config": {"workload": "64 GRAYS clips x 8 frames, resolution in {640x360, 1280x720, 1920x1080, "
+                               "3840x2160}, radius U{0,1,2}, sigma U[5,25], block_step=4, bm_range=9, "
+                               "ps_num=2, ps_range=4, two passes", "pixels_per_run": px},
+        "speedup_vs_per_clip_fp32": round(res["batch_mixed_mpixel_s"] / res["per_clip_fp32_mpixel_s"], 3),
+        **res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +317,10 @@ def main():
                     help="after the run, fill and copy a buffer of this many MiB (a known-size "
                          "reference for HBM byte counters)")
     ap.add_argument("--no-compare", action="store_true", help="time only --precision")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the plans' default launch shapes instead of autotuned ones")
+    ap.add_argument("--workload", default="vbm3d", choices=["vbm3d", "batch"],
+                    help="vbm3d: the 1080p two-pass V-BM3D clip (default); batch: the 64-clip mix")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
 
@@ -214,6 +337,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        run_batch(args)
+        return
     torch.cuda.set_device(0)
     nf = args.frames
     t0 = time.perf_counter()
@@ -222,7 +348,7 @@ def main():
     src = torch.from_numpy(noisy).cuda()
 
     paths = [args.precision] if args.no_compare else ["fp32", "mixed"]
-    res = {p: time_path(core, _capi, src, p, args.steps, args.warmup) for p in paths}
+    res = {p: time_path(core, _capi, src, p, args.steps, args.warmup, not args.no_autotune) for p in paths}
     main_r = res[args.precision]
 
     quality = {}
@@ -255,7 +381,8 @@ def main():
         "data": "synthetic (seeded sinusoids+rectangles, integer motion, N(0,15/255) noise)",
         "config": {"workload": "GRAYS 1920x1080 x30 frames, two-pass BM3Dv2 radius=1, sigma=15, "
                                "block_step=4, bm_range=9, ps_num=2, ps_range=4",
-                   "frames": nf, "precision": args.precision},
+                   "frames": nf, "precision": args.precision,
+                   "launch_shapes": res[args.precision]["tuning"]},
         "mpixel_per_s": round(main_r["fps"] * W * H / 1e6, 2),
         "roofline": {
             "bound": "mfma",

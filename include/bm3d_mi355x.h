@@ -105,7 +105,8 @@ int bm3d_execute(bm3d_plan *plan, float *d_res, long long res_frame_stride, long
                  const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count, void *stream);
 
 /* BM3D followed by VAggregate (the BM3Dv2 composition, source/source.cpp:
- * 944-988) for output frames first_frame .. first_frame+count-1, radius > 0:
+ * 944-988) for output frames first_frame .. first_frame+count-1 (for
+ * radius 0, BM3D's aggregated output, as bm3d_execute + bm3d_aggregate):
  * dst frame i, plane p at dst + i * dst_frame_stride + p * dst_plane_stride,
  * rows dst_stride floats apart (0 = packed, width floats). The same values
  * as bm3d_execute into stacks followed by bm3d_vaggregate (bit for bit when
@@ -216,6 +217,51 @@ int bm3d_profile_read(int kernel, double *total_ms, long long *launches);
  * state algorithmic work; no counterpart in the reference. */
 int bm3d_count_candidates(bm3d_plan *plan, const bm3d_clip *src, const bm3d_clip *ref, int first_frame,
                           int count, unsigned long long *ncand, void *stream);
+
+/* ---- batches of clips ------------------------------------------------------
+ * Many clips of different sizes, radii and parameters denoised together. Each
+ * stage (fixed-point range, block matching, filtering, VAggregate) runs as
+ * one launch over every clip of a wave of clips, the waves bounded by a
+ * device-memory budget; bm3d_batch_create captures the whole sequence, both
+ * passes of every clip, in one HIP graph and bm3d_batch_execute replays it.
+ * No counterpart in the reference, whose plugin launches one graph per frame
+ * of one clip (source/source.cpp:299-441, source/kernel.cu:759-864). */
+
+/* One plane group of one clip (one plane, or the three planes of a YUV444
+ * clip with desc.chroma = 1). Output semantics are BM3Dv2's
+ * (source/source.cpp:911-989): BM3D followed by VAggregate for radius > 0,
+ * BM3D's aggregated output for radius 0; planes not processed (sigma below
+ * FLT_EPSILON, source/source.cpp:916-930) are copied from src. */
+typedef struct bm3d_batch_job {
+    bm3d_desc desc;   /* parameters; desc.final_ is ignored */
+    int passes;       /* 1: out = BM3Dv2(src);
+                         2: basic = BM3Dv2(src), out = BM3Dv2(src, ref = basic) */
+    bm3d_clip src;    /* device clip, rows desc.stride floats apart */
+    float *basic;     /* passes == 2: device frames of the basic estimate, with
+                         src's frame and plane strides */
+    float *out;       /* device output frames, src's frame and plane strides */
+} bm3d_batch_job;
+
+typedef struct bm3d_batch bm3d_batch;
+
+/* Plans every job on the current device (the reference's argument checks
+ * and messages, as bm3d_plan_create), packs the jobs into waves whose
+ * scratch stays within scratch_budget bytes (0: 16 GiB; a job larger than
+ * the budget gets a wave of its own), and captures the graph. The jobs'
+ * buffers must stay valid until bm3d_batch_destroy. */
+int bm3d_batch_create(const bm3d_batch_job *jobs, int njobs, long long scratch_budget, bm3d_batch **batch);
+/* Replays the batch's graph on `stream` (asynchronous). */
+int bm3d_batch_execute(bm3d_batch *batch, void *stream);
+/* Launch shapes for every job (batch_frames and chunks are not used: a job's
+ * frames are evaluated in one run); the graph is captured again. */
+int bm3d_batch_set_tuning(bm3d_batch *batch, const bm3d_tuning *t);
+int bm3d_batch_get_tuning(const bm3d_batch *batch, bm3d_tuning *t);
+/* Times candidate launch shapes by replaying the batch (which writes the
+ * jobs' outputs) and keeps the fastest. Synchronises `stream`. */
+int bm3d_batch_autotune(bm3d_batch *batch, void *stream);
+/* Waves, kernel launches in the graph, and device bytes of scratch. */
+int bm3d_batch_info(const bm3d_batch *batch, int *waves, int *launches, long long *scratch_bytes);
+void bm3d_batch_destroy(bm3d_batch *batch);
 
 /* ---- plugin-level filters ------------------------------------------------
  * The three filters the bm3dcuda plugin registers (source/source.cpp:

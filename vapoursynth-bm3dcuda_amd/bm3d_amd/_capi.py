@@ -76,6 +76,10 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchJob(ctypes.Structure):
+    _fields_ = [("desc", Desc), ("passes", c_int), ("src", Clip), ("basic", c_void_p), ("out", c_void_p)]
+
+
 class BM3DError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(message)
@@ -115,6 +119,20 @@ def lib():
     L.bm3d_plan_set_tuning.restype = c_int
     L.bm3d_plan_autotune.argtypes = [P, ctypes.POINTER(Clip), ctypes.POINTER(Clip), c_int, c_int, P]
     L.bm3d_plan_autotune.restype = c_int
+    L.bm3d_batch_create.argtypes = [ctypes.POINTER(BatchJob), c_int, c_ll, ctypes.POINTER(P)]
+    L.bm3d_batch_create.restype = c_int
+    L.bm3d_batch_execute.argtypes = [P, P]
+    L.bm3d_batch_execute.restype = c_int
+    L.bm3d_batch_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.bm3d_batch_get_tuning.restype = c_int
+    L.bm3d_batch_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.bm3d_batch_set_tuning.restype = c_int
+    L.bm3d_batch_autotune.argtypes = [P, P]
+    L.bm3d_batch_autotune.restype = c_int
+    L.bm3d_batch_info.argtypes = [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_ll)]
+    L.bm3d_batch_info.restype = c_int
+    L.bm3d_batch_destroy.argtypes = [P]
+    L.bm3d_batch_destroy.restype = None
     L.bm3d_execute.argtypes = [P, P, c_ll, c_ll, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
                                c_int, c_int, P]
     L.bm3d_execute.restype = c_int
@@ -178,6 +196,8 @@ EXPORTS = (
     "bm3d_plan_get_tuning", "bm3d_plan_set_tuning", "bm3d_plan_autotune", "bm3d_last_error", "bm3d_version",
     "bm3d_profile_enable",
     "bm3d_profile_reset", "bm3d_profile_read", "bm3d_count_candidates",
+    "bm3d_batch_create", "bm3d_batch_execute", "bm3d_batch_get_tuning", "bm3d_batch_set_tuning",
+    "bm3d_batch_autotune", "bm3d_batch_info", "bm3d_batch_destroy",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
     "bm3d_filter_get_frames", "bm3d_filter_destroy",
 )

@@ -31,6 +31,22 @@ def clip_desc(t):
     return C.Clip(ctypes.c_void_p(t.data_ptr()), p * h * w, h * w, n)
 
 
+def _desc(width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2, ps_range=4, chroma=False,
+          final=False, extractor_exp=0, precision="fp32", stride=None):
+    sig = list(sigma) if isinstance(sigma, (list, tuple)) else [sigma]
+    sig = (sig + [sig[-1]] * 3)[:3]
+    d = C.Desc()
+    d.width, d.height = int(width), int(height)
+    d.stride = int(stride if stride is not None else width)
+    for i in range(3):
+        d.sigma[i] = float(sig[i])
+    d.block_step, d.bm_range, d.radius = int(block_step), int(bm_range), int(radius)
+    d.ps_num, d.ps_range = int(ps_num), int(ps_range)
+    d.chroma, d.final_, d.extractor_exp = int(bool(chroma)), int(bool(final)), int(extractor_exp)
+    d.precision = {"fp32": C.PRECISION_FP32, "mixed": C.PRECISION_MIXED}[precision]
+    return d
+
+
 class Plan:
     """One plane group's baked parameters (the role of get_graphexec,
     source/kernel.cu:759-864)."""
@@ -38,17 +54,8 @@ class Plan:
     def __init__(self, width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2,
                  ps_range=4, chroma=False, final=False, extractor_exp=0, precision="fp32",
                  stride=None):
-        sig = list(sigma) if isinstance(sigma, (list, tuple)) else [sigma]
-        sig = (sig + [sig[-1]] * 3)[:3]
-        d = C.Desc()
-        d.width, d.height = int(width), int(height)
-        d.stride = int(stride if stride is not None else width)
-        for i in range(3):
-            d.sigma[i] = float(sig[i])
-        d.block_step, d.bm_range, d.radius = int(block_step), int(bm_range), int(radius)
-        d.ps_num, d.ps_range = int(ps_num), int(ps_range)
-        d.chroma, d.final_, d.extractor_exp = int(bool(chroma)), int(bool(final)), int(extractor_exp)
-        d.precision = {"fp32": C.PRECISION_FP32, "mixed": C.PRECISION_MIXED}[precision]
+        d = _desc(width, height, sigma, block_step, bm_range, radius, ps_num, ps_range, chroma, final,
+                  extractor_exp, precision, stride)
         self.desc = d
         self.nplanes = 3 if chroma else 1
         self.radius = int(radius)
@@ -187,6 +194,84 @@ class Plan:
                                    ctypes.byref(r) if r else None, int(first), int(count),
                                    _stream_ptr(stream)))
         return g
+
+
+class Batch:
+    """Many clips denoised together (bm3d_batch_*): each job is a dict with
+    the Plan parameters (width, height, sigma, block_step, bm_range, radius,
+    ps_num, ps_range, chroma, extractor_exp, precision), `src` (a clip
+    tensor), `passes` (1: BM3Dv2; 2: two-pass BM3Dv2) and optional `out` /
+    `basic` tensors shaped like src (allocated when absent). execute()
+    replays the batch's captured graph."""
+
+    def __init__(self, jobs, scratch_budget=0):
+        arr = (C.BatchJob * len(jobs))()
+        self.jobs = []
+        for i, j in enumerate(jobs):
+            src = j["src"]
+            _check_clip(src, "clip")
+            n, planes, h, w = src.shape
+            passes = int(j.get("passes", 2))
+            out = j.get("out")
+            if out is None:
+                out = torch.empty_like(src)
+            basic = j.get("basic")
+            if basic is None and passes == 2:
+                basic = torch.empty_like(src)
+            plan_kw = {k: j[k] for k in ("sigma", "block_step", "bm_range", "radius", "ps_num", "ps_range",
+                                         "chroma", "extractor_exp", "precision") if k in j}
+            d = _desc(w, h, **plan_kw)
+            arr[i].desc = d
+            arr[i].passes = passes
+            arr[i].src = clip_desc(src)
+            arr[i].basic = ctypes.c_void_p(basic.data_ptr() if basic is not None else 0)
+            arr[i].out = ctypes.c_void_p(out.data_ptr())
+            self.jobs.append(dict(src=src, out=out, basic=basic, passes=passes))
+        h = ctypes.c_void_p()
+        C.check(C.lib().bm3d_batch_create(arr, len(jobs), int(scratch_budget), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            C.lib().bm3d_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, stream=None):
+        C.check(C.lib().bm3d_batch_execute(self._h, _stream_ptr(stream)))
+        return [j["out"] for j in self.jobs]
+
+    @property
+    def tuning(self):
+        t = C.Tuning()
+        C.check(C.lib().bm3d_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = C.Tuning()
+        C.check(C.lib().bm3d_batch_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        C.check(C.lib().bm3d_batch_set_tuning(self._h, ctypes.byref(t)))
+        return self.tuning
+
+    def autotune(self, stream=None):
+        C.check(C.lib().bm3d_batch_autotune(self._h, _stream_ptr(stream)))
+        return self.tuning
+
+    @property
+    def info(self):
+        waves, launches, scratch = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        C.check(C.lib().bm3d_batch_info(self._h, ctypes.byref(waves), ctypes.byref(launches),
+                                        ctypes.byref(scratch)))
+        return {"waves": waves.value, "launches": launches.value, "scratch_bytes": scratch.value}
 
 
 def unpack_groups(g):

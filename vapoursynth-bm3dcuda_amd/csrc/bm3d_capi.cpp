@@ -84,44 +84,6 @@ struct ProfScope {
 
 }  // namespace
 
-struct bm3d_plan {
-    bm3d_desc d{};
-    float sigma[3]{};   // scaled thresholds
-    float extractor = 0.0f;
-    int nplanes = 1, tw = 1;
-    int nbx = 0, nby = 0, nblocks = 0;
-    int margin = 0;     // search reach: bm_range + radius * ps_range
-    long long plane_size = 0, stack_size = 0;
-    int device = 0;
-    int max_lds = 0;
-    bm3d_tuning tune{};
-    // matching work-group tile: tby rows of 16 reference blocks plus the margin
-    int tby = 16, tile_w = 0, tile_h = 0, tile_wq = 0, tile_hp = 0;
-    size_t lds_bytes = 0;
-    // filter work-group tiling and LDS accumulation region (margin frm)
-    int ftbx = 8, ftby = 8, fntx = 0, fnty = 0, frm = 0, frw = 0, frh = 0;
-    size_t filter_lds_bytes = 0;
-    long long region_ints = 0;  // one region per tile and plane, all tiles: per slot or output frame
-    // device buffers, capacities in elements
-    uint32_t* groups = nullptr;
-    size_t groups_cap = 0;
-    float* corner = nullptr;
-    size_t corner_cap = 0;
-    int* scratch = nullptr;
-    size_t scratch_cap = 0;
-    // fixed-point unit of the accumulation regions: 2^fx_exp, scaled by the
-    // input's range (measured into *range per call) unless extractor_exp is set
-    int fx_exp = 0;
-    float* range = nullptr;
-    // mixed-precision matching kernel (precision MIXED)
-    bool use_dx = false, dx_ok = false;
-    int tile_hp2 = 0;
-    size_t dx_lds_bytes = 0;
-    // mixed-precision filter kernel (precision MIXED): per-lane transform operands
-    bool use_mx_filter = false;
-    void* mx_ops = nullptr;
-};
-
 namespace {
 
 int count_positions(int n, int bs) {
@@ -371,9 +333,8 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch) {
 // Block matching for frames first .. first + count - 1 into `groups` (or
 // counting candidates into `ncand`): the mixed-precision kernel when the
 // plan selected it, else the FP32 kernel.
-hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long long* ncand,
-                           const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
-                           hipStream_t stream) {
+bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long long* ncand, const bm3d_clip* src,
+                           const bm3d_clip* ref, int first_frame) {
     const bm3d_desc& d = p->d;
     const float* clip = d.final_ ? ref->data : src->data;  // matching runs on ref in the final pass
     bm3d::MatchArgs m{};
@@ -389,10 +350,15 @@ hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
     m.groups = groups;
     m.ncand = ncand;
-    if (p->use_dx) {
-        m.tile_hp2 = p->tile_hp2;
-        return bm3d::launch_match_dx(m, p->tby, p->dx_lds_bytes, count, stream);
-    }
+    m.tile_hp2 = p->tile_hp2;
+    return m;
+}
+
+hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long long* ncand,
+                           const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
+                           hipStream_t stream) {
+    const bm3d::MatchArgs m = match_args(p, groups, ncand, src, ref, first_frame);
+    if (p->use_dx) return bm3d::launch_match_dx(m, p->tby, p->dx_lds_bytes, count, stream);
     return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
 }
 
@@ -447,6 +413,29 @@ hipError_t launch_filtering(const bm3d_plan* p, const bm3d::FilterArgs& fa, int 
                                p->filter_lds_bytes, stream);
 }
 
+// VAggregate-from-targets arguments over the plan's buffers (ring frames);
+// the destination fields are the caller's.
+bm3d::VTargetArgs vtarget_args(const bm3d_plan* p, int ring) {
+    const bm3d_desc& d = p->d;
+    bm3d::VTargetArgs va{};
+    va.scratch = p->scratch;
+    va.target_stride = p->region_ints;
+    va.corner = p->corner;
+    va.corner_frame = (long long)p->nplanes * p->plane_size;
+    va.ring = ring;
+    va.fx_range = fx_range(p);
+    va.fx_exp = p->fx_exp;
+    va.W = d.width; va.H = d.height; va.stride = d.stride;
+    va.plane_size = p->plane_size;
+    va.nplanes = p->nplanes;
+    va.plane_mask = 0;
+    for (int i = 0; i < p->nplanes; ++i)
+        if (!(d.chroma && p->sigma[i] < std::numeric_limits<float>::epsilon())) va.plane_mask |= 1u << i;
+    va.bs = d.block_step; va.tbx = p->ftbx; va.tby = p->ftby; va.ntx = p->fntx; va.nty = p->fnty;
+    va.rm = p->frm; va.rw = p->frw; va.rh = p->frh;
+    return va;
+}
+
 // Slot-mode evaluation of frames n0 .. n0 + nf - 1 (stacks path): group
 // lists, slot regions and corner maps per evaluation.
 hipError_t evaluate_slots(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int n0, int nf, float* res,
@@ -476,6 +465,20 @@ int contributors_lo(int t, int r) { return std::max(t - r, 0); }
 int contributors_hi(int t, int r, int N) { return std::min(t + r, N - 1); }
 
 }  // namespace
+
+namespace bm3d {
+
+int plan_apply_tuning(bm3d_plan* p, const bm3d_tuning& t) { return apply_tuning(p, t); }
+const char* plan_tuning_error(const bm3d_tuning& t) { return tuning_range_error(t); }
+MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
+    return match_args(p, nullptr, nullptr, src, ref, 0);
+}
+FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
+    return filter_args(p, src, ref);
+}
+VTargetArgs plan_vtarget_args(const bm3d_plan* p) { return vtarget_args(p, 1); }
+
+}  // namespace bm3d
 
 extern "C" {
 
@@ -868,7 +871,6 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
         return fail(BM3D_EINVAL, "BM3D: \"ref\" must have the layout and number of frames of \"clip\"");
     if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
         return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
-    if (d.radius <= 0) return fail(BM3D_EINVAL, "BM3D: bm3d_execute_vagg needs radius > 0");
     if (p->frm < p->margin)
         return fail(BM3D_EUNSUPPORTED, "BM3D: search reach too large for region aggregation "
                                        "(use bm3d_execute and bm3d_vaggregate)");
@@ -903,25 +905,10 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     fa.target_stride = p->region_ints;
     fa.corner_frame_stride = corner_target;
 
-    bm3d::VTargetArgs va{};
-    va.scratch = p->scratch;
-    va.target_stride = p->region_ints;
-    va.corner = p->corner;
-    va.corner_frame = corner_target;
-    va.ring = ring;
-    va.fx_range = fx_range(p);
-    va.fx_exp = p->fx_exp;
+    bm3d::VTargetArgs va = vtarget_args(p, ring);
     va.dst_frame_stride = dst_frame_stride;
     va.dst_plane_stride = dst_plane_stride;
     va.dst_stride = dst_stride;
-    va.W = d.width; va.H = d.height; va.stride = d.stride;
-    va.plane_size = p->plane_size;
-    va.nplanes = p->nplanes;
-    va.plane_mask = 0;
-    for (int i = 0; i < p->nplanes; ++i)
-        if (!(d.chroma && p->sigma[i] < std::numeric_limits<float>::epsilon())) va.plane_mask |= 1u << i;
-    va.bs = d.block_step; va.tbx = p->ftbx; va.tby = p->ftby; va.ntx = p->fntx; va.nty = p->fnty;
-    va.rm = p->frm; va.rw = p->frw; va.rh = p->frh;
 
     BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), stream));
     int next_out = first_frame;

@@ -96,6 +96,34 @@ template <typename T> __device__ __forceinline__ T lane_xor4(T v) { return dpp_m
 // Value of lane k of the quad (quad_perm [k, k, k, k]).
 template <int kLane, typename T> __device__ __forceinline__ T quad_bcast(T v) { return dpp_mov<kLane * 0x55>(v); }
 
+// Work-group coordinates of a kernel body: blockIdx in a single-job launch,
+// the coordinates within its job's grid in a multi-job launch.
+struct Wg {
+    int x, y, z;
+};
+__device__ __forceinline__ Wg launch_wg() { return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z}; }
+
+// Job of flat work-group index wg (binary search over the sorted begins;
+// every value is uniform over the work-group, so this runs on the scalar
+// unit).
+__device__ __forceinline__ int find_job(const JobGrid* __restrict__ grids, int njobs, int wg) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (grids[mid].begin <= wg) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ Wg job_wg(const JobGrid& g, int wg) {
+    const int local = wg - g.begin;
+    const int plane = g.gx * g.gy;
+    const int z = local / plane;
+    const int rem = local - z * plane;
+    const int y = rem / g.gx;
+    return {rem - y * g.gx, y, z};
+}
+
 // Number of i in [0, 2r] for which VAggregate's output frame
 // n = clamp(m - r + z, 0, N - 1) reads slot z of the evaluation of frame m:
 // it reads slot clamp(2r - i, n - N + 1 + r, n + r) of the evaluation of
@@ -128,17 +156,17 @@ struct SlotTable {
 // regions to scratch: per evaluation (mode 0), or per output frame once the
 // evaluations that add to it are done (mode 1). LDS: tw regions of rh x rw.
 template <typename Body>
-__device__ __forceinline__ void filter_walk(const FilterArgs& a, int* acc, SlotTable& st, int nplanes, bool chroma,
-                                            Body&& body) {
+__device__ __forceinline__ void filter_walk(const FilterArgs& a, Wg w, int* acc, SlotTable& st, int nplanes,
+                                            bool chroma, Body&& body) {
     const int r = a.radius, tw = 2 * r + 1, N = a.num_frames;
     const int regsize = a.rh * a.rw;
-    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int tx = w.x, ty = w.y;
     int e0, e1;
     if (a.mode == 0) {
-        e0 = a.first_frame + (int)blockIdx.z;
+        e0 = a.first_frame + w.z;
         e1 = e0 + 1;
     } else {
-        e0 = a.eval_lo + (int)blockIdx.z * a.chunk;
+        e0 = a.eval_lo + w.z * a.chunk;
         e1 = min(e0 + a.chunk, a.eval_hi);
         if (e0 >= e1) return;  // uniform over the work-group
     }
@@ -153,7 +181,7 @@ __device__ __forceinline__ void filter_walk(const FilterArgs& a, int* acc, SlotT
                 if (a.mode == 0) {
                     st.region[z] = z * regsize;
                     st.mult[z] = 1;
-                    st.corner[z] = (long long)blockIdx.z * a.corner_frame_stride + (long long)(p * tw + z) * a.plane_size;
+                    st.corner[z] = (long long)w.z * a.corner_frame_stride + (long long)(p * tw + z) * a.plane_size;
                 } else {
                     const int t = clampi(m - r + z, 0, N - 1);
                     st.region[z] = (t % tw) * regsize;
@@ -162,7 +190,7 @@ __device__ __forceinline__ void filter_walk(const FilterArgs& a, int* acc, SlotT
                 }
             }
             __syncthreads();
-            body(p, m, a.mode == 0 ? (int)blockIdx.z : m - a.eval_lo);
+            body(p, m, a.mode == 0 ? w.z : m - a.eval_lo);
             __syncthreads();
             if (a.mode == 1) {
                 // frames whose every contributing evaluation (within r) is done
@@ -184,7 +212,7 @@ __device__ __forceinline__ void filter_walk(const FilterArgs& a, int* acc, SlotT
         }
         if (a.mode == 0) {
             __syncthreads();
-            int* out = a.scratch + ((((size_t)blockIdx.z * a.nty + ty) * a.ntx + tx) * nplanes + p) * tw * regsize;
+            int* out = a.scratch + ((((size_t)w.z * a.nty + ty) * a.ntx + tx) * nplanes + p) * tw * regsize;
             for (int i = threadIdx.x; i < tw * regsize; i += blockDim.x) out[i] = acc[i];
         }
         __syncthreads();

@@ -171,7 +171,7 @@ __device__ __forceinline__ float row_sum16(float s) {
 // scratch layout, and the same accumulation modes, as bm3d_filter
 // (bm3d_kernels.hip, filter_walk).
 template <bool kFinal, bool kChroma>
-__global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
+__device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     extern __shared__ float lds[];
     __shared__ SlotTable st;
     int* acc = reinterpret_cast<int*>(lds);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int q = lane >> 4, c = lane & 15;
-    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int tx = wg.x, ty = wg.y;
     const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
     const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
     const int ngroups = a.tbx * a.tby;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
     MxOps o;
     o.load(reinterpret_cast<const uint4*>(a.mx_ops), lane);
 
-    filter_walk(a, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
+    filter_walk(a, wg, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
         const float* src = a.src + (size_t)p * a.plane_stride;
         float* res = a.res ? a.res + (size_t)gf * a.res_frame_stride + (size_t)p * a.res_plane_stride : nullptr;
@@ -325,6 +325,19 @@ __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
     });
 }
 
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
+    filter_mx_body<kFinal, kChroma>(a, launch_wg());
+}
+
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(512) void bm3d_filter_mx_multi(const FilterArgs* __restrict__ jobs,
+                                                            const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const FilterArgs& a = jobs[j];
+    filter_mx_body<kFinal, kChroma>(a, job_wg(grids[j], blockIdx.x));
+}
+
 }  // namespace
 
 // 8-point DCT-II, 4 x orthonormal: D[0][x] = sqrt(2), D[m][x] = 2 cos((2x + 1) m pi / 16).
@@ -371,12 +384,33 @@ hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int g
     return hipGetLastError();
 }
 
+hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
+                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s) {
+    dim3 grid(wgs), block(64 * waves);
+    if (final_) {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
+    } else {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
+    }
+    return hipGetLastError();
+}
+
 hipError_t set_filter_mx_lds_limit(size_t bytes) {
     const void* ks[] = {
         reinterpret_cast<const void*>(&bm3d_filter_mx<false, false>),
         reinterpret_cast<const void*>(&bm3d_filter_mx<false, true>),
         reinterpret_cast<const void*>(&bm3d_filter_mx<true, false>),
         reinterpret_cast<const void*>(&bm3d_filter_mx<true, true>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, false>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, true>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, false>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, true>),
     };
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

@@ -20,6 +20,15 @@ constexpr int kMaxRadius = 15;
 // Largest temporal width 2r + 1 (the filters' per-slot tables).
 constexpr int kMaxSlots = 2 * kMaxRadius + 1;
 
+// Multi-job launches (one launch over several clips' work): a 1-D grid
+// whose work-groups are the concatenated grids of the jobs. Job j's grid is
+// gx x gy x gz work-groups starting at flat index `begin`; entries are
+// sorted by begin and the launch has jobs[n-1].begin + its size work-groups.
+struct JobGrid {
+    int begin;
+    int gx, gy;
+};
+
 struct MatchArgs {
     const float* clip;       // plane used for matching, frame 0
     long long frame_stride;  // floats between frames
@@ -152,6 +161,27 @@ struct VAggArgs {
 };
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
+// Multi-job forms: `jobs` and `grids` are device arrays of njobs entries;
+// `wgs` work-groups in all; every job shares the launch's block size (tby
+// rows, waves) and kernel variant; lds_bytes covers the largest job.
+hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+                              size_t lds_bytes, hipStream_t s);
+hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+                                 size_t lds_bytes, int ps_num, hipStream_t s);
+hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
+                               bool chroma, int waves, size_t lds_bytes, hipStream_t s);
+hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
+                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s);
+hipError_t launch_vagg_targets_multi(const VTargetArgs* jobs, const JobGrid* grids, int njobs, int wgs,
+                                     hipStream_t s);
+// Largest |sample| per job: grid of job j = (min(H, 64), frames * nplanes).
+struct AbsmaxArgs {
+    const float* clip;
+    long long frame_stride, plane_stride;
+    int nplanes, W, H, stride;
+    float* out;
+};
+hipError_t launch_absmax_multi(const AbsmaxArgs* jobs, const JobGrid* grids, int njobs, int wgs, hipStream_t s);
 hipError_t set_match_lds_limit(size_t bytes);
 // Mixed-precision matching (bm3d_match_dx.hip): same launch geometry as
 // launch_match; LDS bytes as returned by dx_lds_bytes.

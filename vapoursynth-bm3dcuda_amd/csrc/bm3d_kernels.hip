@@ -108,17 +108,17 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
 // with kCount, also adds the number of in-plane candidates to *a.ncand.
 template <bool kCount>
-__global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
+__device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     extern __shared__ float lds[];
     const int tby = blockDim.x >> 4;
     const int lx = threadIdx.x & 15;
     const int ly = threadIdx.x >> 4;
-    const int f = blockIdx.z;
+    const int f = w.z;
     const int n = a.first_frame + f;
     const int r = a.radius;
 
-    const int bx0 = blockIdx.x * 16;
-    const int by0 = blockIdx.y * tby;
+    const int bx0 = w.x * 16;
+    const int by0 = w.y * tby;
     const int bxi = bx0 + lx;
     const int byi = by0 + ly;
     const bool active = bxi < a.nbx && byi < a.nby;
@@ -212,6 +212,19 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
     }
 }
 
+template <bool kCount>
+__global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
+    match_body<kCount>(a, launch_wg());
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(256) void bm3d_match_multi(const MatchArgs* __restrict__ jobs,
+                                                        const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const MatchArgs& a = jobs[j];
+    match_body<kCount>(a, job_wg(grids[j], blockIdx.x));
+}
+
 // ---------------------------------------------------------------- filtering
 
 // 8x8 transpose across the 8 lanes of a group: lane k holds x[j] = M[j][k]
@@ -281,7 +294,7 @@ __device__ __forceinline__ void inverse3d(float (&v)[8][8], int k) {
 // outside the regions (mode 0 only) go to global float atomics on the
 // stack. Member weights go to a corner map (one atomic per member).
 template <bool kFinal, bool kChroma>
-__global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
+__device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
     extern __shared__ float lds[];
     __shared__ SlotTable st;
     int* acc = reinterpret_cast<int*>(lds);
@@ -290,14 +303,14 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 3;
     const int k = lane & 7;
-    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int tx = wg.x, ty = wg.y;
     const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;
     const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
     const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
 
-    filter_walk(a, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
+    filter_walk(a, wg, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
         const float* src = a.src + (size_t)p * a.plane_stride;
         float* res = a.res ? a.res + (size_t)gf * a.res_frame_stride + (size_t)p * a.res_plane_stride : nullptr;
@@ -434,6 +447,19 @@ __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
     });
 }
 
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
+    filter_body<kFinal, kChroma>(a, launch_wg());
+}
+
+template <bool kFinal, bool kChroma>
+__global__ __launch_bounds__(256) void bm3d_filter_multi(const FilterArgs* __restrict__ jobs,
+                                                         const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const FilterArgs& a = jobs[j];
+    filter_body<kFinal, kChroma>(a, job_wg(grids[j], blockIdx.x));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- elementwise
@@ -450,13 +476,13 @@ __global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
 // Largest finite |sample| over the planes of `frames` frames: grid (rows,
 // frames * nplanes), 256 threads over the columns; one atomic per wave on
 // the float's bit pattern (non-negative floats order like their bits).
-__global__ __launch_bounds__(256) void bm3d_absmax(const float* __restrict__ clip, long long frame_stride,
-                                                   long long plane_stride, int nplanes, int W, int H, int stride,
-                                                   float* out) {
-    const int f = blockIdx.y / nplanes, p = blockIdx.y % nplanes;
+__device__ __forceinline__ void absmax_body(const float* __restrict__ clip, long long frame_stride,
+                                            long long plane_stride, int nplanes, int W, int H, int stride,
+                                            float* out, Wg w, int rows_step) {
+    const int f = w.y / nplanes, p = w.y % nplanes;
     const float* base = clip + (size_t)f * frame_stride + (size_t)p * plane_stride;
     float m = 0.0f;
-    for (int y = blockIdx.x; y < H; y += gridDim.x) {
+    for (int y = w.x; y < H; y += rows_step) {
         const float* row = base + (size_t)y * stride;
         for (int x = threadIdx.x; x < W; x += blockDim.x) {
             const float v = fabsf(row[x]);
@@ -470,6 +496,20 @@ __global__ __launch_bounds__(256) void bm3d_absmax(const float* __restrict__ cli
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
     if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(256) void bm3d_absmax(const float* __restrict__ clip, long long frame_stride,
+                                                   long long plane_stride, int nplanes, int W, int H, int stride,
+                                                   float* out) {
+    absmax_body(clip, frame_stride, plane_stride, nplanes, W, H, stride, out, launch_wg(), gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void bm3d_absmax_multi(const AbsmaxArgs* __restrict__ jobs,
+                                                         const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const AbsmaxArgs& a = jobs[j];
+    absmax_body(a.clip, a.frame_stride, a.plane_stride, a.nplanes, a.W, a.H, a.stride, a.out,
+                job_wg(grids[j], blockIdx.x), grids[j].gx);
 }
 
 // Weighted-sum rows of the stacks: the integer sum of the tile regions
@@ -629,15 +669,15 @@ __device__ __forceinline__ void covering_tiles(int v, int span, float inv_span, 
 // the unit; den = the corner map's 8 x 8 box sum (rows, then columns, as
 // bm3d_den_box); out = num / den. Work-group: 64 x 8 pixels, grid
 // (ceil(W/64), ceil(H/8), count * nplanes).
-__global__ __launch_bounds__(256) void bm3d_vagg_targets(VTargetArgs a) {
+__device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
     __shared__ float cm[15][72];      // corner-map window: rows y0-7.., columns x0-7..
     __shared__ float rowsum[15][64];
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // 0..3
-    const int x0 = blockIdx.x * 64;
+    const int x0 = w.x * 64;
     const int x = x0 + tx;
-    const int y0 = blockIdx.y * 8;
-    const int fo = blockIdx.z / a.nplanes, p = blockIdx.z % a.nplanes;
+    const int y0 = w.y * 8;
+    const int fo = w.z / a.nplanes, p = w.z % a.nplanes;
     if (!((a.plane_mask >> p) & 1u)) return;  // uniform per work-group
     const int n = a.first_frame + fo;
     const int slot = n % a.ring;
@@ -690,6 +730,15 @@ __global__ __launch_bounds__(256) void bm3d_vagg_targets(VTargetArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void bm3d_vagg_targets(VTargetArgs a) { vagg_targets_body(a, launch_wg()); }
+
+__global__ __launch_bounds__(256) void bm3d_vagg_targets_multi(const VTargetArgs* __restrict__ jobs,
+                                                               const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const VTargetArgs& a = jobs[j];
+    vagg_targets_body(a, job_wg(grids[j], blockIdx.x));
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
@@ -700,9 +749,16 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
     return hipGetLastError();
 }
 
+hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+                              size_t lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL((bm3d_match_multi<false>), dim3(wgs), dim3(16 * tby), lds_bytes, s, jobs, grids, njobs);
+    return hipGetLastError();
+}
+
 hipError_t set_match_lds_limit(size_t bytes) {
     const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<false>),
-                         reinterpret_cast<const void*>(&bm3d_match<true>)};
+                         reinterpret_cast<const void*>(&bm3d_match<true>),
+                         reinterpret_cast<const void*>(&bm3d_match_multi<false>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
@@ -724,11 +780,28 @@ hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int grid
     return hipGetLastError();
 }
 
+hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
+                               bool chroma, int waves, size_t lds_bytes, hipStream_t s) {
+    dim3 grid(wgs), block(64 * waves);
+    if (final_) {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_multi<true, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else hipLaunchKernelGGL((bm3d_filter_multi<true, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
+    } else {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_multi<false, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else hipLaunchKernelGGL((bm3d_filter_multi<false, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
+    }
+    return hipGetLastError();
+}
+
 hipError_t set_filter_lds_limit(size_t bytes) {
     const void* fns[] = {reinterpret_cast<const void*>(&bm3d_filter<true, true>),
                          reinterpret_cast<const void*>(&bm3d_filter<true, false>),
                          reinterpret_cast<const void*>(&bm3d_filter<false, true>),
-                         reinterpret_cast<const void*>(&bm3d_filter<false, false>)};
+                         reinterpret_cast<const void*>(&bm3d_filter<false, false>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, true>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, false>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, true>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, false>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
@@ -767,6 +840,17 @@ hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s) {
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
     dim3 grid((a.W + 255) / 256, a.H, count);
     hipLaunchKernelGGL(bm3d_divide, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vagg_targets_multi(const VTargetArgs* jobs, const JobGrid* grids, int njobs, int wgs,
+                                     hipStream_t s) {
+    hipLaunchKernelGGL(bm3d_vagg_targets_multi, dim3(wgs), dim3(256), 0, s, jobs, grids, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax_multi(const AbsmaxArgs* jobs, const JobGrid* grids, int njobs, int wgs, hipStream_t s) {
+    hipLaunchKernelGGL(bm3d_absmax_multi, dim3(wgs), dim3(256), 0, s, jobs, grids, njobs);
     return hipGetLastError();
 }
 

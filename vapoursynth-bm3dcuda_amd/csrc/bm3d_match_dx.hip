@@ -348,16 +348,16 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
 // One lane per reference block; grid (ceil(nbx / 16), ceil(nby / TBY),
 // frames). kKt: keys kept per temporal frame (>= 2 ps_num).
 template <bool kCount, int kKt>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx(MatchArgs a) {
+__device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
     __shared__ uint32_t centre_s[8][256];  // the spatial list, per lane
     const int tby = blockDim.x >> 4;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int f = blockIdx.z;
+    const int f = w.z;
     const int n = a.first_frame + f;
     const int r = a.radius;
-    const int bx0 = blockIdx.x * 16, by0 = blockIdx.y * tby;
+    const int bx0 = w.x * 16, by0 = w.y * tby;
     const int bxi = bx0 + lx, byi = by0 + ly;
     const bool active = bxi < a.nbx && byi < a.nby;
     const int x = min(min(bxi, a.nbx - 1) * a.bs, a.W - 8);
@@ -519,6 +519,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     }
 }
 
+template <bool kCount, int kKt>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx(MatchArgs a) {
+    match_dx_body<kCount, kKt>(a, launch_wg());
+}
+
+template <int kKt>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx_multi(
+    const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const MatchArgs& a = jobs[j];
+    match_dx_body<false, kKt>(a, job_wg(grids[j], blockIdx.x));
+}
+
 namespace {
 template <int kKt>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
@@ -529,9 +542,16 @@ hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, 
     return hipGetLastError();
 }
 template <int kKt>
+hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+                           size_t lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt>), dim3(wgs), dim3(16 * tby), lds_bytes, s, jobs, grids, njobs);
+    return hipGetLastError();
+}
+template <int kKt>
 hipError_t set_dx_limit(size_t bytes) {
     const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt>)};
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
@@ -547,6 +567,15 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
         case 4: return launch_dx<4>(a, tby, lds_bytes, frames, s);
         case 8: return launch_dx<8>(a, tby, lds_bytes, frames, s);
         default: return launch_dx<16>(a, tby, lds_bytes, frames, s);
+    }
+}
+
+hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+                                 size_t lds_bytes, int ps_num, hipStream_t s) {
+    switch (dx_temporal_keys(ps_num)) {
+        case 4: return launch_dx_multi<4>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
+        case 8: return launch_dx_multi<8>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
+        default: return launch_dx_multi<16>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
     }
 }
 
