@@ -176,7 +176,7 @@ int bm3d_plan_select_kernels(bm3d_plan *plan, int matching, int filter);
  * differs. bm3d_plan_create picks defaults; bm3d_plan_autotune measures
  * candidates on the caller's data. */
 typedef struct bm3d_tuning {
-    int match_rows;     /* rows of 16 reference blocks per matching work-group: 4, 8 or 16 */
+    int match_rows;     /* rows of reference blocks per matching work-group: 4, 8 or 16 */
     int filter_tile_x;  /* reference blocks per filtering work-group tile: 1..32 each, */
     int filter_tile_y;  /*   at most 256 in all                                         */
     int filter_waves;   /* waves (64 lanes) per filtering work-group: 1..8 (the FP32
@@ -185,6 +185,8 @@ typedef struct bm3d_tuning {
     int chunks;         /* bm3d_execute_vagg: runs of evaluations per filtering tile and
                            launch, 1..batch_frames (more runs: more work-groups, more
                            regions summed with atomics) */
+    int match_width;    /* reference blocks per row of a matching work-group: 16 or 32
+                           (match_width x match_rows <= 256) */
 } bm3d_tuning;
 
 int bm3d_plan_get_tuning(const bm3d_plan *plan, bm3d_tuning *out);

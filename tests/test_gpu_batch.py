@@ -110,7 +110,8 @@ def test_batch_shapes_and_autotune_keep_results():
     batch = core.Batch(jobs)
     batch.execute()
     want = [bj["out"].cpu().numpy() for bj in batch.jobs]
-    for t in [dict(match_rows=4), dict(filter_tile_x=16, filter_tile_y=8, filter_waves=8),
+    for t in [dict(match_rows=4), dict(match_width=32, match_rows=8),
+              dict(filter_tile_x=16, filter_tile_y=8, filter_waves=8),
               dict(filter_tile_x=4, filter_tile_y=4, filter_waves=1)]:
         batch.set_tuning(**t)
         batch.execute()

@@ -43,6 +43,8 @@ TUNINGS = [
     dict(filter_tile_x=3, filter_tile_y=5, filter_waves=1),
     dict(match_rows=4),
     dict(match_rows=8, filter_tile_x=16, filter_tile_y=16),
+    dict(match_width=32, match_rows=8),
+    dict(match_width=32, match_rows=4, filter_tile_x=16, filter_tile_y=8),
 ]
 
 
@@ -94,6 +96,8 @@ def test_stack_shapes_equal(name):
 
 @pytest.mark.parametrize("bad,msg", [
     (dict(match_rows=5), "match_rows"),
+    (dict(match_width=8), "match_width"),
+    (dict(match_width=32, match_rows=16), "at most 256"),
     (dict(filter_tile_x=0), "filter tile"),
     (dict(filter_tile_x=32, filter_tile_y=16), "filter tile"),
     (dict(filter_waves=9), "filter_waves"),
@@ -120,7 +124,8 @@ def test_autotune_keeps_results():
     plan = _plan(core, W, H, prm, "mixed")
     want = plan.execute_vagg(src).cpu().numpy()
     t = plan.autotune(src)
-    assert set(t) == {"match_rows", "filter_tile_x", "filter_tile_y", "filter_waves", "batch_frames", "chunks"}
+    assert set(t) == {"match_rows", "filter_tile_x", "filter_tile_y", "filter_waves", "batch_frames", "chunks",
+                      "match_width"}
     np.testing.assert_array_equal(plan.execute_vagg(src).cpu().numpy(), want)
     # a new plan with the same parameters starts from the tuned shapes
     assert _plan(core, W, H, prm, "mixed").tuning == t

@@ -70,7 +70,7 @@ class Frames(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("match_rows", c_int), ("filter_tile_x", c_int), ("filter_tile_y", c_int),
-                ("filter_waves", c_int), ("batch_frames", c_int), ("chunks", c_int)]
+                ("filter_waves", c_int), ("batch_frames", c_int), ("chunks", c_int), ("match_width", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

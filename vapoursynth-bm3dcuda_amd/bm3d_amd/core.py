@@ -142,7 +142,7 @@ class Plan:
     @property
     def tuning(self):
         """Launch shapes as a dict (match_rows, filter_tile_x, filter_tile_y,
-        filter_waves, batch_frames, chunks)."""
+        filter_waves, batch_frames, chunks, match_width)."""
         t = C.Tuning()
         C.check(C.lib().bm3d_plan_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()

@@ -57,7 +57,7 @@ struct Launch {
     int kind;  // 0 absmax, 1 match, 2 mixed match, 3 filter, 4 mixed filter, 5 vagg
     std::vector<int> jobs;
     int pass = 0;
-    int block = 0;  // tby (matching) or waves (filtering)
+    int block = 0;  // threads (matching) or waves (filtering)
     int ps_num = 0, final_ = 0, chroma = 0;
     size_t lds = 0;
     size_t args_off = 0, grids_off = 0;  // in the table buffer
@@ -210,7 +210,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int keys = p->use_dx ? bm3d::dx_temporal_keys(p->d.ps_num) : 0;
-                mgroups[{p->use_dx ? 1 : 0, keys, p->tby}].push_back(i);
+                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby}].push_back(i);
             }
             for (auto& [key, ids] : mgroups) {
                 Launch l{};
@@ -226,7 +226,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     bm3d::MatchArgs a = bm3d::plan_match_args(p, &j.job.src, &ref);
                     a.groups = reinterpret_cast<uint32_t*>(base + j.groups_off);
                     a.first_frame = 0;
-                    const int gx = (p->nbx + 15) / 16, gy = (p->nby + p->tby - 1) / p->tby;
+                    const int gx = (p->nbx + p->tbw - 1) / p->tbw, gy = (p->nby + p->tby - 1) / p->tby;
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy * j.job.src.num_frames;
                     l.lds = std::max(l.lds, p->use_dx ? p->dx_lds_bytes : p->lds_bytes);
@@ -537,9 +537,11 @@ int bm3d_batch_autotune(bm3d_batch* b, void* stream_) {
                 best = t;
             }
         };
-        for (int rows : {16, 8, 4}) {
+        const int shapes[][2] = {{16, 16}, {16, 8}, {16, 4}, {32, 8}, {32, 4}};
+        for (auto& sh : shapes) {
             bm3d_tuning t = best;
-            t.match_rows = rows;
+            t.match_width = sh[0];
+            t.match_rows = sh[1];
             consider(t);
         }
         const int tiles[][3] = {{8, 8, 4}, {8, 8, 8}, {16, 8, 4}, {16, 8, 8}, {8, 16, 8}, {16, 16, 8}, {8, 4, 4}};

@@ -128,6 +128,7 @@ hipError_t raise_lds_limit(int family, size_t bytes) {
 bm3d_tuning default_tuning(const bm3d_plan* p) {
     bm3d_tuning t{};
     t.match_rows = 16;
+    t.match_width = 16;
     t.filter_tile_x = 8;
     t.filter_tile_y = 8;
     t.filter_waves = p->d.precision == BM3D_PRECISION_MIXED ? 8 : 4;
@@ -138,6 +139,8 @@ bm3d_tuning default_tuning(const bm3d_plan* p) {
 
 const char* tuning_range_error(const bm3d_tuning& t) {
     if (t.match_rows != 4 && t.match_rows != 8 && t.match_rows != 16) return "match_rows must be 4, 8 or 16";
+    if (t.match_width != 16 && t.match_width != 32) return "match_width must be 16 or 32";
+    if (t.match_width * t.match_rows > 256) return "match_width x match_rows must be at most 256";
     if (t.filter_tile_x < 1 || t.filter_tile_x > 32 || t.filter_tile_y < 1 || t.filter_tile_y > 32 ||
         t.filter_tile_x * t.filter_tile_y > 256)
         return "filter tile must be 1..32 blocks per side and at most 256 blocks";
@@ -182,8 +185,9 @@ void load_tune_file(TuneCache& c) {
         std::string key(line, colon - line);
         while (!key.empty() && key.back() == ' ') key.pop_back();
         bm3d_tuning t{};
-        if (std::sscanf(colon + 1, "%d %d %d %d %d %d", &t.match_rows, &t.filter_tile_x, &t.filter_tile_y,
-                        &t.filter_waves, &t.batch_frames, &t.chunks) == 6 && !tuning_range_error(t))
+        if (std::sscanf(colon + 1, "%d %d %d %d %d %d %d", &t.match_rows, &t.filter_tile_x, &t.filter_tile_y,
+                        &t.filter_waves, &t.batch_frames, &t.chunks, &t.match_width) == 7 &&
+            !tuning_range_error(t))
             c.entries.emplace_back(key, t);
     }
     std::fclose(f);
@@ -219,21 +223,23 @@ void store_tuning(const bm3d_plan* p, const bm3d_tuning& t) {
     FILE* f = std::fopen(path, "w");
     if (!f) return;
     for (auto& e : c.entries)
-        std::fprintf(f, "%s : %d %d %d %d %d %d\n", e.first.c_str(), e.second.match_rows, e.second.filter_tile_x,
-                     e.second.filter_tile_y, e.second.filter_waves, e.second.batch_frames, e.second.chunks);
+        std::fprintf(f, "%s : %d %d %d %d %d %d %d\n", e.first.c_str(), e.second.match_rows, e.second.filter_tile_x,
+                     e.second.filter_tile_y, e.second.filter_waves, e.second.batch_frames, e.second.chunks,
+                     e.second.match_width);
     std::fclose(f);
 }
 
-// Matching tile for a work-group of 16 x rows reference blocks; false when
-// its LDS does not fit the device.
-bool match_geometry(bm3d_plan* p, int rows) {
+// Matching tile for a work-group of width x rows reference blocks; false
+// when its LDS does not fit the device.
+bool match_geometry(bm3d_plan* p, int rows, int width) {
     const int bs = p->d.block_step;
-    const int tile_w = 15 * bs + 8 + 2 * p->margin;
+    const int tile_w = (width - 1) * bs + 8 + 2 * p->margin;
     const int tile_h = (rows - 1) * bs + 8 + 2 * p->margin;
     const int tile_wq = (tile_w + bs - 1) / bs;
     const int tile_hp = tile_h | 1;  // odd column pitch: lanes of a row hit distinct banks
     const size_t bytes = (size_t)bs * tile_wq * tile_hp * sizeof(float);
     if (bytes > (size_t)p->max_lds) return false;
+    p->tbw = width;
     p->tby = rows;
     p->tile_w = tile_w;
     p->tile_h = tile_h;
@@ -245,8 +251,8 @@ bool match_geometry(bm3d_plan* p, int rows) {
 
 // Mixed-precision matching kernel (bm3d_match_dx.hip): the FP32 kernel's
 // tile, stored as two planes of FP16 pairs (hp2 words per column, odd so
-// that the 16 lanes of a row hit distinct banks). Scan indices are packed
-// in 12 bits.
+// that the lanes of a row hit distinct banks). Scan indices are packed in
+// 12 bits.
 void dx_geometry(bm3d_plan* p) {
     const bm3d_desc& d = p->d;
     p->dx_ok = false;
@@ -281,7 +287,7 @@ bool filter_geometry(bm3d_plan* p, int tbx, int tby, int rm) {
 // with a reduced margin (members beyond it go to global atomics, and
 // bm3d_execute_vagg is unavailable).
 int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
-    if (!match_geometry(p, t.match_rows) && !match_geometry(p, 4))
+    if (!match_geometry(p, t.match_rows, t.match_width) && !match_geometry(p, 4, 16))
         return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
     dx_geometry(p);
     if (!p->dx_ok) p->use_dx = false;
@@ -298,6 +304,7 @@ int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
     p->tune = t;
     p->tune.match_rows = p->tby;
+    p->tune.match_width = p->tbw;
     p->tune.filter_tile_x = p->ftbx;
     p->tune.filter_tile_y = p->ftby;
     return BM3D_OK;
@@ -347,6 +354,7 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.ps_num = d.ps_num; m.ps_range = d.ps_range;
     m.nbx = p->nbx; m.nby = p->nby; m.nblocks = p->nblocks;
     m.margin = p->margin;
+    m.tbw = p->tbw;
     m.tile_w = p->tile_w; m.tile_h = p->tile_h; m.tile_wq = p->tile_wq; m.tile_hp = p->tile_hp;
     m.groups = groups;
     m.ncand = ncand;
@@ -792,9 +800,11 @@ int bm3d_plan_autotune(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref,
     if (rc == BM3D_OK) {
         // coordinate descent: matching rows, then filter tile and waves,
         // then batch size and evaluation runs
-        for (int rows : {16, 8, 4}) {
+        const int shapes[][2] = {{16, 16}, {16, 8}, {16, 4}, {32, 8}, {32, 4}};
+        for (auto& sh : shapes) {
             bm3d_tuning t = best;
-            t.match_rows = rows;
+            t.match_width = sh[0];
+            t.match_rows = sh[1];
             consider(t);
         }
         const int tiles[][3] = {{8, 8, 4}, {8, 8, 8}, {16, 8, 4}, {16, 8, 8}, {8, 16, 8}, {16, 16, 8},

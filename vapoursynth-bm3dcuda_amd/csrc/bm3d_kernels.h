@@ -38,6 +38,7 @@ struct MatchArgs {
     int bs, bm_range, radius, ps_num, ps_range;
     int nbx, nby, nblocks;
     int margin;              // bm_range + radius * ps_range
+    int tbw;                 // work-group width in reference blocks (16 or 32)
     int tile_w, tile_h, tile_wq, tile_hp;
     int tile_hp2;            // mixed kernel: FP16 pair words per tile column
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
@@ -160,13 +161,15 @@ struct VAggArgs {
     int radius, num_frames, first_frame;
 };
 
+// Block size tbw x tby (a.tbw blocks wide, tby rows).
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 // Multi-job forms: `jobs` and `grids` are device arrays of njobs entries;
-// `wgs` work-groups in all; every job shares the launch's block size (tby
-// rows, waves) and kernel variant; lds_bytes covers the largest job.
-hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+// `wgs` work-groups in all; every job shares the launch's block size
+// (threads: tbw x tby for matching, 64 x waves for filtering) and kernel
+// variant; lds_bytes covers the largest job.
+hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, hipStream_t s);
-hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                  size_t lds_bytes, int ps_num, hipStream_t s);
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, int waves, size_t lds_bytes, hipStream_t s);

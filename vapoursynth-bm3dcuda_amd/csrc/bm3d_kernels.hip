@@ -100,9 +100,9 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
 
 // ---------------------------------------------------------------- matching
 
-// Block matching: one lane per reference block; a work-group covers 16 x TBY
-// reference blocks (TBY = blockDim.x / 16). Grid: (ceil(nbx/16),
-// ceil(nby/TBY), frames). Spatial search on the centre slot (source/kernel.cu
+// Block matching: one lane per reference block; a work-group covers
+// TBW x TBY reference blocks (TBW = a.tbw, 16 or 32; TBY = blockDim.x /
+// TBW). Grid: (ceil(nbx/TBW), ceil(nby/TBY), frames). Spatial search on the centre slot (source/kernel.cu
 // :432-492), then for r > 0 the predictive temporal search
 // (source/kernel.cu:495-614, cpu_source/source.cpp:220-321), then the centre
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
@@ -110,14 +110,15 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
 template <bool kCount>
 __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     extern __shared__ float lds[];
-    const int tby = blockDim.x >> 4;
-    const int lx = threadIdx.x & 15;
-    const int ly = threadIdx.x >> 4;
+    const int wshift = a.tbw == 32 ? 5 : 4;
+    const int tby = blockDim.x >> wshift;
+    const int lx = threadIdx.x & (a.tbw - 1);
+    const int ly = threadIdx.x >> wshift;
     const int f = w.z;
     const int n = a.first_frame + f;
     const int r = a.radius;
 
-    const int bx0 = w.x * 16;
+    const int bx0 = w.x * a.tbw;
     const int by0 = w.y * tby;
     const int bxi = bx0 + lx;
     const int byi = by0 + ly;
@@ -742,16 +743,16 @@ __global__ __launch_bounds__(256) void bm3d_vagg_targets_multi(const VTargetArgs
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
-    dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
-    dim3 block(16 * tby);
+    dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
+    dim3 block(a.tbw * tby);
     if (a.ncand) hipLaunchKernelGGL((bm3d_match<true>), grid, block, lds_bytes, s, a);
     else hipLaunchKernelGGL((bm3d_match<false>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL((bm3d_match_multi<false>), dim3(wgs), dim3(16 * tby), lds_bytes, s, jobs, grids, njobs);
+    hipLaunchKernelGGL((bm3d_match_multi<false>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids, njobs);
     return hipGetLastError();
 }
 

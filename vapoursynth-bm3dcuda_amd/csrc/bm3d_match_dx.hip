@@ -16,7 +16,7 @@
 //
 //   D  = <r', c'>            32 v_dot2c_f32_f16 (FP32 accumulation)
 //   N  = |c'|^2              sliding sum of per-column v_dot2c sums
-//   A  = |r'|^2 + N - 2 D    the squared distance up to FP16 rounding
+//   A  = (|r'|^2 + N) / 2 - D  half the squared distance up to FP16 rounding
 //
 // A is packed with the candidate's scan index into a 32-bit key (A's
 // exponent and 11 mantissa bits above a 12-bit index) and kept by a
@@ -58,12 +58,13 @@ struct KeyList {
 #pragma unroll
         for (int i = 0; i < K; ++i) k[i] = 0xFFFFFFFFu;
     }
+    // (each entry is updated in place, so the list keeps its registers)
     __device__ __forceinline__ void push(uint32_t key) {
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            const uint32_t lo = min(k[i], key);
-            key = max(k[i], key);
-            k[i] = lo;
+            const uint32_t hi = max(k[i], key);
+            k[i] = min(k[i], key);
+            key = hi;
         }
     }
     // removes and returns the smallest key
@@ -168,7 +169,8 @@ __device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const 
     load_pair_col(ring[(U + 7) & 7], lds + addr);
     csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
     N += csq[(U + 7) & 7];
-    float d0 = 0.0f, d1 = 0.0f;
+    // half the distance: (|r'|^2 + |c'|^2) / 2 - <r', c'>, rp holding -r'
+    float d0 = fma_(N, 0.5f, nr), d1 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; c += 2) {
         const int s0 = (U + c) & 7, s1 = (U + c + 1) & 7;
@@ -178,7 +180,8 @@ __device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const 
             d1 = dot2(rp[c + 1][i], ring[s1][i], d1);
         }
     }
-    const float approx = fmaxf(fma_(-2.0f, d0 + d1, N + nr), 0.0f);
+    // clamped at zero: a rounded negative would order after every key
+    const float approx = fmaxf(d0 + d1, 0.0f);
     N -= csq[U & 7];  // column leaving the window of the next candidate
     const bool ok = (unsigned)u - vlo <= vspan;
     if constexpr (kCount) cnt += ok ? 1u : 0u;
@@ -327,7 +330,8 @@ __device__ void exact_window(const float* __restrict__ rplane, int x, int y, con
     }
 }
 
-// r' = r - mu as packed vertical pairs: rp[c][i] = (r'[2i][c], r'[2i+1][c]).
+// r' = r - mu as negated packed vertical pairs: rp[c][i] = -(r'[2i][c],
+// r'[2i+1][c]) (the scan accumulates -<r', c'>); returns |r'|^2 / 2.
 __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __restrict__ plane, int stride, int x,
                                            int y, float mu) {
     float nr = 0.0f;
@@ -337,27 +341,29 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
         for (int i = 0; i < 4; ++i) {
             const _Float16 a = (_Float16)(plane[(size_t)(y + 2 * i) * stride + x + c] - mu);
             const _Float16 b = (_Float16)(plane[(size_t)(y + 2 * i + 1) * stride + x + c] - mu);
-            rp[c][i] = f16x2{a, b};
+            rp[c][i] = f16x2{-a, -b};
             nr = dot2(rp[c][i], rp[c][i], nr);
         }
-    return nr;
+    return 0.5f * nr;
 }
 
 }  // namespace
 
-// One lane per reference block; grid (ceil(nbx / 16), ceil(nby / TBY),
-// frames). kKt: keys kept per temporal frame (>= 2 ps_num).
+// One lane per reference block; a work-group covers TBW x TBY blocks (TBW =
+// a.tbw, 16 or 32; TBY = blockDim.x / TBW); grid (ceil(nbx / TBW),
+// ceil(nby / TBY), frames). kKt: keys kept per temporal frame (>= 2 ps_num).
 template <bool kCount, int kKt>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
     __shared__ uint32_t centre_s[8][256];  // the spatial list, per lane
-    const int tby = blockDim.x >> 4;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int wshift = a.tbw == 32 ? 5 : 4;
+    const int tby = blockDim.x >> wshift;
+    const int lx = threadIdx.x & (a.tbw - 1), ly = threadIdx.x >> wshift;
     const int f = w.z;
     const int n = a.first_frame + f;
     const int r = a.radius;
-    const int bx0 = w.x * 16, by0 = w.y * tby;
+    const int bx0 = w.x * a.tbw, by0 = w.y * tby;
     const int bxi = bx0 + lx, byi = by0 + ly;
     const bool active = bxi < a.nbx && byi < a.nby;
     const int x = min(min(bxi, a.nbx - 1) * a.bs, a.W - 8);
@@ -535,16 +541,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
 namespace {
 template <int kKt>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
-    dim3 grid((a.nbx + 15) / 16, (a.nby + tby - 1) / tby, frames);
-    dim3 block(16 * tby);
+    dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
+    dim3 block(a.tbw * tby);
     if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt>), grid, block, lds_bytes, s, a);
     else hipLaunchKernelGGL((bm3d_match_dx<false, kKt>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
 template <int kKt>
-hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                            size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt>), dim3(wgs), dim3(16 * tby), lds_bytes, s, jobs, grids, njobs);
+    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids, njobs);
     return hipGetLastError();
 }
 template <int kKt>
@@ -570,12 +576,12 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
     }
 }
 
-hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int tby,
+hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                  size_t lds_bytes, int ps_num, hipStream_t s) {
     switch (dx_temporal_keys(ps_num)) {
-        case 4: return launch_dx_multi<4>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
-        case 8: return launch_dx_multi<8>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
-        default: return launch_dx_multi<16>(jobs, grids, njobs, wgs, tby, lds_bytes, s);
+        case 4: return launch_dx_multi<4>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        case 8: return launch_dx_multi<8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        default: return launch_dx_multi<16>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
     }
 }
 
