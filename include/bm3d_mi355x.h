@@ -51,6 +51,22 @@ extern "C" {
 #define BM3D_PRECISION_MIXED 1 /* FP16 block-matching pre-scoring, exact
                                   FP32 rescoring of the best 2k             */
 
+/* Block distance (the bm_error_s option of the NVRTC variant,
+ * rtc_source/kernel.hpp:363-504; parsing rtc_source/source.cpp:854-875). */
+#define BM3D_BM_SSD 0       /* sum of squared differences (the default)    */
+#define BM3D_BM_SAD 1       /* sum of absolute differences                 */
+#define BM3D_BM_ZSSD 2      /* SSD after removing the block means          */
+#define BM3D_BM_ZSAD 3      /* SAD after removing the block means          */
+#define BM3D_BM_SSD_NORM 4  /* 2 - 2 <c, n> / (|c| |n| + FLT_EPSILON)     */
+
+/* 8-point transforms, each scaled to 4 x its orthonormal form (the
+ * transform_2d_s / transform_1d_s options, rtc_source/kernel.hpp:103-361;
+ * parsing rtc_source/source.cpp:877-912). */
+#define BM3D_TRANSFORM_DCT 0      /* DCT-II / DCT-III (the default)        */
+#define BM3D_TRANSFORM_HAAR 1     /* Haar, with the reference's inverse    */
+#define BM3D_TRANSFORM_WHT 2      /* Walsh-Hadamard, sequency order        */
+#define BM3D_TRANSFORM_BIOR1_5 3  /* biorthogonal spline 1.5               */
+
 typedef struct bm3d_plan bm3d_plan;
 
 /* Parameters of one plane group: one plane, or the three planes of a
@@ -71,6 +87,12 @@ typedef struct bm3d_desc {
     int final_;          /* 0: basic (hard threshold); 1: final (Wiener, needs ref) */
     int extractor_exp;   /* 0 = off, else pre-rounding with 2^extractor_exp */
     int precision;       /* BM3D_PRECISION_* */
+    /* Options of the NVRTC variant (0 = the bm3dcuda behaviour). The mixed
+     * precision matcher serves BM3D_BM_SSD only; other distances run the FP32
+     * matcher. */
+    int bm_error;        /* BM3D_BM_* */
+    int transform_2d;    /* BM3D_TRANSFORM_*: along block rows and columns */
+    int transform_1d;    /* BM3D_TRANSFORM_*: along the group's members    */
 } bm3d_desc;
 
 /* A clip in device memory: frame f, plane p starts at
@@ -307,6 +329,14 @@ typedef struct bm3d_args {
     int n_extractor_exp;  long long extractor_exp;
     int n_zero_init;      long long zero_init;
     int n_precision;      long long precision;
+    /* The NVRTC variant's string options (rtc_source/source.cpp:854-912):
+     * per plane, case-insensitive, missing entries repeat the previous one;
+     * bm_error_s in {"ssd", "sad", "zssd", "zsad", "ssd/norm"} (default
+     * "ssd"), transform_*_s in {"dct", "haar", "wht", "bior1.5"} (default
+     * "dct"). With chroma, plane 0's entries apply to all planes. */
+    int n_bm_error_s;     const char *bm_error_s[3];
+    int n_transform_2d_s; const char *transform_2d_s[3];
+    int n_transform_1d_s; const char *transform_1d_s[3];
 } bm3d_args;
 
 /* A clip's planes in device memory: frame f of plane p starts at

@@ -59,6 +59,33 @@ void SFX(oracle_dct8)(T *v, int n, int inverse) {
     }
 }
 
+/* Transform `kind` (0 dct, 1 haar, 2 wht, 3 bior1.5) of one 8-vector. The
+ * DCT takes its butterfly; the others multiply by their matrix, each output
+ * as a product with v[0] followed by an FMA chain over v[1] .. v[7] (zero
+ * entries included), the order the HIP kernels use. The reference's NVRTC
+ * kernels evaluate the same linear maps by butterflies
+ * (rtc_source/kernel.hpp:200-361) under -use_fast_math
+ * (rtc_source/source.cpp:262-267); results agree to float rounding. */
+static inline void SFX(xform8)(T v[8], int kind, int inverse) {
+    if (kind == 0) {
+        if (inverse) SFX(dct8_inv)(v);
+        else SFX(dct8_fwd)(v);
+        return;
+    }
+    const double *M = xform_matrix(kind, inverse);
+    T o[8];
+    for (int r = 0; r < 8; ++r) {
+        T acc = (T)M[r * 8] * v[0];
+        for (int k = 1; k < 8; ++k) acc = FMA((T)M[r * 8 + k], v[k], acc);
+        o[r] = acc;
+    }
+    for (int r = 0; r < 8; ++r) v[r] = o[r];
+}
+
+void SFX(oracle_xform8)(T *v, int n, int kind, int inverse) {
+    for (int i = 0; i < n; ++i) SFX(xform8)(&v[8 * i], kind, inverse);
+}
+
 /* 3-D transform of a group g[i][j][k] (member i, row j, column k).
  * Forward: along j for every (i,k), then along k for every (i,j), then along
  * i -- the order of cpu_source/source.cpp:596-603 (kernel.cu:264-270).
@@ -66,43 +93,54 @@ void SFX(oracle_dct8)(T *v, int n, int inverse) {
  * reference inverts in forward order (cpu_source/source.cpp:607-612); the
  * two differ only in float rounding, and the reversed order is the one the
  * HIP kernels use (fewer cross-lane transposes). */
-static void SFX(transform3d)(T g[8][8][8], int inverse) {
+/* t2 is the transform along j and k (transform_2d), t1 along i
+ * (transform_1d), as rtc_source/kernel.hpp:585-605; the bm3dcuda kernels
+ * are the case t2 = t1 = dct. */
+static void SFX(transform3d_kinds)(T g[8][8][8], int inverse, int t2, int t1) {
     T v[8];
     if (!inverse) {
         for (int i = 0; i < 8; ++i)
             for (int k = 0; k < 8; ++k) {
                 for (int j = 0; j < 8; ++j) v[j] = g[i][j][k];
-                SFX(dct8_fwd)(v);
+                SFX(xform8)(v, t2, 0);
                 for (int j = 0; j < 8; ++j) g[i][j][k] = v[j];
             }
         for (int i = 0; i < 8; ++i)
-            for (int j = 0; j < 8; ++j) SFX(dct8_fwd)(g[i][j]);
+            for (int j = 0; j < 8; ++j) SFX(xform8)(g[i][j], t2, 0);
         for (int j = 0; j < 8; ++j)
             for (int k = 0; k < 8; ++k) {
                 for (int i = 0; i < 8; ++i) v[i] = g[i][j][k];
-                SFX(dct8_fwd)(v);
+                SFX(xform8)(v, t1, 0);
                 for (int i = 0; i < 8; ++i) g[i][j][k] = v[i];
             }
     } else {
         for (int j = 0; j < 8; ++j)
             for (int k = 0; k < 8; ++k) {
                 for (int i = 0; i < 8; ++i) v[i] = g[i][j][k];
-                SFX(dct8_inv)(v);
+                SFX(xform8)(v, t1, 1);
                 for (int i = 0; i < 8; ++i) g[i][j][k] = v[i];
             }
         for (int i = 0; i < 8; ++i)
-            for (int j = 0; j < 8; ++j) SFX(dct8_inv)(g[i][j]);
+            for (int j = 0; j < 8; ++j) SFX(xform8)(g[i][j], t2, 1);
         for (int i = 0; i < 8; ++i)
             for (int k = 0; k < 8; ++k) {
                 for (int j = 0; j < 8; ++j) v[j] = g[i][j][k];
-                SFX(dct8_inv)(v);
+                SFX(xform8)(v, t2, 1);
                 for (int j = 0; j < 8; ++j) g[i][j][k] = v[j];
             }
     }
 }
 
+static void SFX(transform3d)(T g[8][8][8], int inverse) {
+    SFX(transform3d_kinds)(g, inverse, 0, 0);
+}
+
 void SFX(oracle_transform3d)(T *g, int inverse) {
     SFX(transform3d)((T (*)[8][8])g, inverse);
+}
+
+void SFX(oracle_transform3d_kinds)(T *g, int inverse, int t2, int t1) {
+    SFX(transform3d_kinds)((T (*)[8][8])g, inverse, t2, t1);
 }
 
 /* ---- block distance -------------------------------------------------------
@@ -138,6 +176,77 @@ T SFX(oracle_ssd)(const T *ref, const T *p, int stride) {
     return SFX(ssd)(ref, p, stride);
 }
 
+/* Pairwise tree over the 8 per-column partial results, the order of
+ * reduce_subwarp (rtc_source/kernel.hpp:363-370) and reduce_add
+ * (cpu_source/source.cpp:98-103). */
+static inline T SFX(tree8)(const T s[8]) {
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+/* Block mean, rtc_source/kernel.hpp:413-421: per column a pairwise sum of
+ * its 8 rows, the column tree, times 1/64. */
+static inline T SFX(block_mean)(const T *b, int stride) {
+    T s[8];
+    for (int c = 0; c < 8; ++c)
+        s[c] = ((b[0 * stride + c] + b[1 * stride + c]) + (b[2 * stride + c] + b[3 * stride + c])) +
+               ((b[4 * stride + c] + b[5 * stride + c]) + (b[6 * stride + c] + b[7 * stride + c]));
+    return SFX(tree8)(s) * ((T)1 / (T)64);
+}
+
+/* The block distances of the NVRTC variant (rtc_source/kernel.hpp:372-504):
+ * 0 ssd, 1 sad, 2 zssd, 3 zsad, 4 ssd/norm. Per column an even-row and an
+ * odd-row accumulation (errors[i % 2] over rows i, starting from 0, with the
+ * products fused as nvcc contracts them), the column sum e0 + e1, then the
+ * pairwise tree over columns. Means subtract as (c - n) - (mean_c - mean_n).
+ * ssd/norm: 2 - (2 <c, n>) / fma(|c|, |n|, FLT_EPSILON) with correctly
+ * rounded sqrt and division (the reference's -use_fast_math versions are
+ * approximations of these). */
+static inline T SFX(block_cost)(const T *ref /* 8x8 contiguous */, const T *p, int stride, int cost) {
+    if (cost == 0) return SFX(ssd)(ref, p, stride);
+    T s[8];
+    if (cost == 4) {
+        T sc[8], sn[8];
+        for (int c = 0; c < 8; ++c) {
+            T a[2] = {0, 0}, b[2] = {0, 0}, d[2] = {0, 0};
+            for (int j = 0; j < 8; ++j) {
+                T cv = ref[j * 8 + c], nv = p[j * stride + c];
+                if (j < 2) {
+                    a[j] = cv * cv;
+                    b[j] = nv * nv;
+                    d[j] = cv * nv;
+                } else {
+                    a[j & 1] = FMA(cv, cv, a[j & 1]);
+                    b[j & 1] = FMA(nv, nv, b[j & 1]);
+                    d[j & 1] = FMA(cv, nv, d[j & 1]);
+                }
+            }
+            sc[c] = a[0] + a[1];
+            sn[c] = b[0] + b[1];
+            s[c] = d[0] + d[1];
+        }
+        T cn = SQRT(SFX(tree8)(sc)), nn = SQRT(SFX(tree8)(sn));
+        T dot = SFX(tree8)(s);
+        return (T)2 - ((T)2 * dot) / FMA(cn, nn, (T)FLT_EPSILON);
+    }
+    T dm = 0;
+    if (cost == 2 || cost == 3) dm = SFX(block_mean)(ref, 8) - SFX(block_mean)(p, stride);
+    for (int c = 0; c < 8; ++c) {
+        T e[2] = {0, 0};
+        for (int j = 0; j < 8; ++j) {
+            T v = ref[j * 8 + c] - p[j * stride + c];
+            if (cost >= 2) v = v - dm;
+            if (cost == 2) e[j & 1] = j < 2 ? v * v : FMA(v, v, e[j & 1]);
+            else e[j & 1] = j < 2 ? FABS(v) : e[j & 1] + FABS(v);
+        }
+        s[c] = e[0] + e[1];
+    }
+    return SFX(tree8)(s);
+}
+
+T SFX(oracle_block_cost)(const T *ref, const T *p, int stride, int cost) {
+    return SFX(block_cost)(ref, p, stride, cost);
+}
+
 /* ---- sorted list of the 8 best matches ---------------------------------- */
 typedef struct {
     T err[8];
@@ -167,14 +276,14 @@ static inline void SFX(list_insert)(SFX(list8) *L, T e, int x, int y, int z) {
  * cpu_source/source.cpp:143-215 (block_matching). */
 static void SFX(search_window)(SFX(list8) *L, const T *refblk, const T *img, int stride,
                                int W, int H, int range, int cx, int cy, int z,
-                               long long *ncand) {
+                               long long *ncand, int cost) {
     int left = cx - range > 0 ? cx - range : 0;
     int right = cx + range < W - 8 ? cx + range : W - 8;
     int top = cy - range > 0 ? cy - range : 0;
     int bottom = cy + range < H - 8 ? cy + range : H - 8;
     for (int row = top; row <= bottom; ++row)
         for (int col = left; col <= right; ++col) {
-            T e = SFX(ssd)(refblk, &img[(size_t)row * stride + col], stride);
+            T e = SFX(block_cost)(refblk, &img[(size_t)row * stride + col], stride, cost);
             SFX(list_insert)(L, e, col, row, z);
         }
     if (ncand) *ncand += (long long)(bottom - top + 1) * (right - left + 1);
@@ -190,7 +299,7 @@ static void SFX(search_window)(SFX(list8) *L, const T *refblk, const T *img, int
  * which only matters when a window holds fewer than 8 candidates. */
 static void SFX(match_block)(SFX(list8) *L, const T *refblk, const T *const *slots, int stride,
                              int W, int H, int R, int radius, int ps_num, int ps_range,
-                             int x, int y, long long *ncand) {
+                             int x, int y, long long *ncand, int cost) {
     const float big = FLT_MAX;
     for (int i = 0; i < 8; ++i) {
         L->err[i] = (T)big;
@@ -198,7 +307,7 @@ static void SFX(match_block)(SFX(list8) *L, const T *refblk, const T *const *slo
         L->y[i] = y;
         L->z[i] = radius;
     }
-    SFX(search_window)(L, refblk, slots[radius], stride, W, H, R, x, y, radius, ncand);
+    SFX(search_window)(L, refblk, slots[radius], stride, W, H, R, x, y, radius, ncand, cost);
     if (radius == 0) return;
     int cxs[8], cys[8];
     for (int i = 0; i < 8; ++i) { cxs[i] = L->x[i]; cys[i] = L->y[i]; }
@@ -214,7 +323,7 @@ static void SFX(match_block)(SFX(list8) *L, const T *refblk, const T *const *slo
                 F.x[i] = 0; F.y[i] = 0; F.z[i] = z;
             }
             for (int i = 0; i < ps_num; ++i)
-                SFX(search_window)(&F, refblk, slots[z], stride, W, H, ps_range, lx[i], ly[i], z, ncand);
+                SFX(search_window)(&F, refblk, slots[z], stride, W, H, ps_range, lx[i], ly[i], z, ncand, cost);
             for (int i = 0; i < ps_num; ++i)
                 SFX(list_insert)(L, F.err[i], F.x[i], F.y[i], z);
             memcpy(lx, F.x, sizeof lx);
@@ -315,10 +424,11 @@ T SFX(oracle_collab_wiener)(T *n, T *b, T sigma) {
  * sigma: already scaled as source/source.cpp:520-523.
  * groups (nullable): per reference block 8 x (x, y, z) after centre insert.
  * Returns the number of block-matching candidates evaluated. */
-long long SFX(oracle_bm3d)(T *res, const T *const *src, const T *const *ref,
-                           int W, int H, int stride, const float *sigma,
-                           int bs, int R, int radius, int ps_num, int ps_range,
-                           int chroma, int final_, float extractor, int *groups) {
+long long SFX(oracle_bm3d_ex)(T *res, const T *const *src, const T *const *ref,
+                              int W, int H, int stride, const float *sigma,
+                              int bs, int R, int radius, int ps_num, int ps_range,
+                              int chroma, int final_, float extractor, int *groups,
+                              int cost, int t2, int t1) {
     const int tw = 2 * radius + 1;
     const int nplanes = chroma ? 3 : 1;
     const size_t plane_sz = (size_t)H * stride;
@@ -340,7 +450,7 @@ long long SFX(oracle_bm3d)(T *res, const T *const *src, const T *const *ref,
                     refblk[j * 8 + k] = bm_slots[radius][(size_t)(y + j) * stride + x + k];
             SFX(list8) L;
             SFX(match_block)(&L, refblk, bm_slots, stride, W, H, R, radius, ps_num, ps_range,
-                             x, y, &ncand);
+                             x, y, &ncand, cost);
             SFX(insert_centre)(&L, x, y, radius);
             if (groups) {
                 for (int i = 0; i < 8; ++i) {
@@ -359,19 +469,19 @@ long long SFX(oracle_bm3d)(T *res, const T *const *src, const T *const *ref,
                         for (int k = 0; k < 8; ++k)
                             grp[i][j][k] = ps[L.z[i]][(size_t)(L.y[i] + j) * stride + L.x[i] + k];
                 T w;
-                SFX(transform3d)(grp, 0);
+                SFX(transform3d_kinds)(grp, 0, t2, t1);
                 if (final_) {
                     const T *const *pr = &ref[p * tw];
                     for (int i = 0; i < 8; ++i)
                         for (int j = 0; j < 8; ++j)
                             for (int k = 0; k < 8; ++k)
                                 grb[i][j][k] = pr[L.z[i]][(size_t)(L.y[i] + j) * stride + L.x[i] + k];
-                    SFX(transform3d)(grb, 0);
+                    SFX(transform3d_kinds)(grb, 0, t2, t1);
                     w = SFX(wiener)(grp, grb, (T)sigma[p]);
                 } else {
                     w = SFX(hard_threshold)(grp, (T)sigma[p]);
                 }
-                SFX(transform3d)(grp, 1);
+                SFX(transform3d_kinds)(grp, 1, t2, t1);
                 /* aggregation with pre-rounding, kernel.cu:717-756 */
                 T *rp = res + (size_t)p * tw * 2 * plane_sz;
                 T wr = (w + E) - E;
@@ -395,10 +505,18 @@ long long SFX(oracle_bm3d)(T *res, const T *const *src, const T *const *ref,
     return ncand;
 }
 
+long long SFX(oracle_bm3d)(T *res, const T *const *src, const T *const *ref,
+                           int W, int H, int stride, const float *sigma,
+                           int bs, int R, int radius, int ps_num, int ps_range,
+                           int chroma, int final_, float extractor, int *groups) {
+    return SFX(oracle_bm3d_ex)(res, src, ref, W, H, stride, sigma, bs, R, radius, ps_num, ps_range,
+                               chroma, final_, extractor, groups, 0, 0, 0);
+}
+
 /* Block matching only: the group lists of every reference block. */
-long long SFX(oracle_match)(const T *const *slots, int W, int H, int stride,
-                            int bs, int R, int radius, int ps_num, int ps_range,
-                            int *groups, T *errors) {
+long long SFX(oracle_match_ex)(const T *const *slots, int W, int H, int stride,
+                               int bs, int R, int radius, int ps_num, int ps_range,
+                               int *groups, T *errors, int cost) {
     long long ncand = 0;
     size_t blk = 0;
     for (int _y = 0; _y < H - 8 + bs; _y += bs) {
@@ -411,7 +529,7 @@ long long SFX(oracle_match)(const T *const *slots, int W, int H, int stride,
                     refblk[j * 8 + k] = slots[radius][(size_t)(y + j) * stride + x + k];
             SFX(list8) L;
             SFX(match_block)(&L, refblk, slots, stride, W, H, R, radius, ps_num, ps_range,
-                             x, y, &ncand);
+                             x, y, &ncand, cost);
             SFX(insert_centre)(&L, x, y, radius);
             for (int i = 0; i < 8; ++i) {
                 groups[blk * 24 + i * 3 + 0] = L.x[i];
@@ -422,4 +540,10 @@ long long SFX(oracle_match)(const T *const *slots, int W, int H, int stride,
         }
     }
     return ncand;
+}
+
+long long SFX(oracle_match)(const T *const *slots, int W, int H, int stride,
+                            int bs, int R, int radius, int ps_num, int ps_range,
+                            int *groups, T *errors) {
+    return SFX(oracle_match_ex)(slots, W, H, stride, bs, R, radius, ps_num, ps_range, groups, errors, 0);
 }

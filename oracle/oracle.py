@@ -40,17 +40,34 @@ def lib():
         for sfx, ct in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
             P = ctypes.POINTER(ct)
             PP = ctypes.POINTER(P)
+            bm3d_args = [P, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
             f = getattr(L, f"oracle_bm3d_{sfx}")
             f.restype = ctypes.c_longlong
-            f.argtypes = [P, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                          ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
-                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                          ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
-            f = getattr(L, f"oracle_match_{sfx}")
+            f.argtypes = bm3d_args
+            f = getattr(L, f"oracle_bm3d_ex_{sfx}")
             f.restype = ctypes.c_longlong
-            f.argtypes = [PP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            f.argtypes = bm3d_args + [ctypes.c_int] * 3
+            match_args = [PP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                           ctypes.POINTER(ctypes.c_int), P]
+            f = getattr(L, f"oracle_match_{sfx}")
+            f.restype = ctypes.c_longlong
+            f.argtypes = match_args
+            f = getattr(L, f"oracle_match_ex_{sfx}")
+            f.restype = ctypes.c_longlong
+            f.argtypes = match_args + [ctypes.c_int]
+            f = getattr(L, f"oracle_xform8_{sfx}")
+            f.restype = None
+            f.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f = getattr(L, f"oracle_transform3d_kinds_{sfx}")
+            f.restype = None
+            f.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f = getattr(L, f"oracle_block_cost_{sfx}")
+            f.restype = ct
+            f.argtypes = [P, P, ctypes.c_int, ctypes.c_int]
             f = getattr(L, f"oracle_dct8_{sfx}")
             f.restype = None
             f.argtypes = [P, ctypes.c_int, ctypes.c_int]
@@ -68,6 +85,8 @@ def lib():
             f = getattr(L, f"oracle_collab_wiener_{sfx}")
             f.restype = ct
             f.argtypes = [P, P, ct]
+        L.oracle_xform_matrix.restype = None
+        L.oracle_xform_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.oracle_preround_f32.restype = ctypes.c_float
         L.oracle_preround_f32.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib = L
@@ -97,22 +116,54 @@ def dct8(v, inverse=False, dtype=np.float32):
     return a.reshape(np.shape(v))
 
 
-def transform3d(g, inverse=False, dtype=np.float32):
-    """3-D transform of one 8x8x8 group [member, row, column]."""
+# Option codes of the NVRTC variant (the C ABI's BM3D_BM_* / BM3D_TRANSFORM_*).
+BM_ERRORS = {"ssd": 0, "sad": 1, "zssd": 2, "zsad": 3, "ssd/norm": 4}
+TRANSFORMS = {"dct": 0, "haar": 1, "wht": 2, "bior1.5": 3}
+
+
+def transform3d(g, inverse=False, dtype=np.float32, t2=0, t1=0):
+    """3-D transform of one 8x8x8 group [member, row, column]: transform kind
+    t2 along rows and columns, t1 along the members."""
     a = np.ascontiguousarray(np.asarray(g, dtype=dtype)).copy()
     assert a.shape == (8, 8, 8)
-    getattr(lib(), f"oracle_transform3d_{_sfx(dtype)}")(_ptr(a), int(inverse))
+    if t2 == 0 and t1 == 0:
+        getattr(lib(), f"oracle_transform3d_{_sfx(dtype)}")(_ptr(a), int(inverse))
+    else:
+        getattr(lib(), f"oracle_transform3d_kinds_{_sfx(dtype)}")(_ptr(a), int(inverse), int(t2), int(t1))
     return a
+
+
+def xform8(v, kind, inverse=False, dtype=np.float32):
+    """8-point transform `kind` (TRANSFORMS) over the last axis."""
+    a = np.ascontiguousarray(np.asarray(v, dtype=dtype).reshape(-1, 8)).copy()
+    getattr(lib(), f"oracle_xform8_{_sfx(dtype)}")(_ptr(a), a.shape[0], int(kind), int(inverse))
+    return a.reshape(np.shape(v))
+
+
+def xform_matrix(kind, inverse=False):
+    """The oracle's 8x8 matrix of transform `kind` (float64)."""
+    out = np.zeros(64, np.float64)
+    lib().oracle_xform_matrix(int(kind), int(inverse), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out.reshape(8, 8)
+
+
+def _at(im, x, y, dtype):
+    P = ctypes.POINTER(ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double)
+    return ctypes.cast(im.ctypes.data + (y * im.shape[1] + x) * im.itemsize, P)
 
 
 def ssd(ref_block, img, x, y, dtype=np.float32):
     r = np.ascontiguousarray(np.asarray(ref_block, dtype=dtype).reshape(8, 8))
     im = np.ascontiguousarray(np.asarray(img, dtype=dtype))
-    off = im[y:, x:]
-    P = ctypes.POINTER(ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double)
-    base = ctypes.cast(im.ctypes.data + (y * im.shape[1] + x) * im.itemsize, P)
-    del off
-    return getattr(lib(), f"oracle_ssd_{_sfx(dtype)}")(_ptr(r), base, im.shape[1])
+    return getattr(lib(), f"oracle_ssd_{_sfx(dtype)}")(_ptr(r), _at(im, x, y, dtype), im.shape[1])
+
+
+def block_cost(ref_block, img, x, y, cost, dtype=np.float32):
+    """Block distance `cost` (BM_ERRORS) between ref_block and img[y:y+8, x:x+8]."""
+    r = np.ascontiguousarray(np.asarray(ref_block, dtype=dtype).reshape(8, 8))
+    im = np.ascontiguousarray(np.asarray(img, dtype=dtype))
+    return getattr(lib(), f"oracle_block_cost_{_sfx(dtype)}")(_ptr(r), _at(im, x, y, dtype), im.shape[1],
+                                                                int(cost))
 
 
 def collab_hard(group, sigma_scaled, dtype=np.float32):
@@ -153,6 +204,9 @@ class Params:
     ps_range: int = 4
     chroma: bool = False
     extractor_exp: int = 0
+    bm_error: int = 0       # BM_ERRORS
+    transform_2d: int = 0   # TRANSFORMS
+    transform_1d: int = 0
 
     @property
     def extractor(self):
@@ -165,7 +219,7 @@ def block_positions(n, bs):
 
 
 # --------------------------------------------------------------- matching
-def match(slots, bs, bm_range, radius=0, ps_num=2, ps_range=4, dtype=np.float32):
+def match(slots, bs, bm_range, radius=0, ps_num=2, ps_range=4, dtype=np.float32, cost=0):
     """Group lists [blocks, 8, 3] (x, y, z) and errors [blocks, 8] for a window
     of 2r+1 planes (slots[r] is the centre). Also returns the candidate count."""
     slots = [np.ascontiguousarray(s, dtype=dtype) for s in slots]
@@ -173,15 +227,15 @@ def match(slots, bs, bm_range, radius=0, ps_num=2, ps_range=4, dtype=np.float32)
     nb = len(block_positions(H, bs)) * len(block_positions(W, bs))
     groups = np.zeros((nb, 8, 3), np.int32)
     errs = np.zeros((nb, 8), dtype)
-    n = getattr(lib(), f"oracle_match_{_sfx(dtype)}")(
+    n = getattr(lib(), f"oracle_match_ex_{_sfx(dtype)}")(
         _ptr_array(slots, dtype), W, H, W, bs, bm_range, radius, ps_num, ps_range,
-        groups.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(errs))
+        groups.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(errs), int(cost))
     return groups, errs, n
 
 
 # ------------------------------------------------------------ BM3D(n)
 def bm3d_window(src_slots, ref_slots, sigma_scaled, bs, bm_range, radius, ps_num, ps_range,
-                chroma, final, extractor=0.0, dtype=np.float32, want_groups=False):
+                chroma, final, extractor=0.0, dtype=np.float32, want_groups=False, options=(0, 0, 0)):
     """One kernel evaluation. src_slots/ref_slots: arrays [planes, 2r+1, H, W].
     Returns res [planes, 2r+1, 2, H, W] (num, den) and optionally groups."""
     src_slots = np.ascontiguousarray(src_slots, dtype=dtype)
@@ -198,10 +252,10 @@ def bm3d_window(src_slots, ref_slots, sigma_scaled, bs, bm_range, radius, ps_num
     nb = len(block_positions(H, bs)) * len(block_positions(W, bs))
     groups = np.zeros((nb, 8, 3), np.int32) if want_groups else None
     gptr = groups.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if want_groups else None
-    n = getattr(lib(), f"oracle_bm3d_{_sfx(dtype)}")(
+    n = getattr(lib(), f"oracle_bm3d_ex_{_sfx(dtype)}")(
         _ptr(res), _ptr_array(src_list, dtype), _ptr_array(ref_list, dtype),
         W, H, W, sig, bs, bm_range, radius, ps_num, ps_range, int(chroma), int(final),
-        float(extractor), gptr)
+        float(extractor), gptr, *[int(o) for o in options])
     if want_groups:
         return res, groups, n
     return res
@@ -224,11 +278,12 @@ def bm3d_frame(clip, ref, n, p: Params, dtype=np.float32):
     process = [s >= FLT_EPSILON for s in sig]
     scaled = [scale_sigma(s, final) for s in sig]
     r = p.radius
+    opts = (p.bm_error, p.transform_2d, p.transform_1d)
     if p.chroma:
         src_w = _window(clip, n, r)
         ref_w = _window(ref, n, r) if final else None
         res = bm3d_window(src_w, ref_w, scaled, p.block_step, p.bm_range, r, p.ps_num,
-                          p.ps_range, True, final, p.extractor, dtype)
+                          p.ps_range, True, final, p.extractor, dtype, options=opts)
         res_all = res
     else:
         res_all = np.zeros((npl, 2 * r + 1, 2, H, W), dtype)
@@ -239,7 +294,7 @@ def bm3d_frame(clip, ref, n, p: Params, dtype=np.float32):
             ref_w = _window(ref[:, pl:pl + 1], n, r) if final else None
             res_all[pl:pl + 1] = bm3d_window(src_w, ref_w, [scaled[pl]], p.block_step,
                                              p.bm_range, r, p.ps_num, p.ps_range, False, final,
-                                             p.extractor, dtype)
+                                             p.extractor, dtype, options=opts)
     if r == 0:
         out = np.array(clip[n], dtype=dtype, copy=True)
         for pl in range(npl):

@@ -120,3 +120,42 @@ def test_vaggregate_rejects():
     assert rc == C.BM3D_EINVAL and "radius > 0" in msg
     rc, msg, _ = vagg([3], vi(h=48 * 6), vi())
     assert rc == C.BM3D_EINVAL and msg == "VAggregate: invalid plane index"
+
+
+# ---- the bm3dcuda_rtc string options (rtc_source/source.cpp:854-912) ----
+RTC_CASES = [
+    (dict(bm_error_s=[b"xyz"]), "BM3D: invalid 'bm_error_s': xyz"),
+    (dict(bm_error_s=[b"SSD", b"Foo"]), "BM3D: invalid 'bm_error_s': foo"),
+    (dict(bm_error_s=[b"ssd_norm"]), "BM3D: invalid 'bm_error_s': ssd_norm"),
+    (dict(bm_error_s=[b""]), "BM3D: invalid 'bm_error_s': "),
+    (dict(transform_2d_s=[b"dct", b"BIOR1.5", b"fft"]), "BM3D: invalid 'transform_2d_s': fft"),
+    (dict(transform_2d_s=[b"bior1_5"]), "BM3D: invalid 'transform_2d_s': bior1_5"),
+    (dict(transform_1d_s=[b"Haar", b"dst"]), "BM3D: invalid 'transform_1d_s': dst"),
+    # bm_error_s is checked first, then transform_2d_s, then transform_1d_s
+    (dict(bm_error_s=[b"x"], transform_2d_s=[b"y"], transform_1d_s=[b"z"]), "BM3D: invalid 'bm_error_s': x"),
+    (dict(transform_2d_s=[b"y"], transform_1d_s=[b"z"]), "BM3D: invalid 'transform_2d_s': y"),
+]
+
+
+@pytest.mark.parametrize("v2", [False, True])
+@pytest.mark.parametrize("kw,msg", RTC_CASES)
+def test_rtc_options_rejected(kw, msg, v2):
+    rc, got = create(args(**kw), vi(), None, v2)
+    assert rc == C.BM3D_EINVAL
+    assert got == msg
+
+
+@pytest.mark.parametrize("kw", [
+    dict(bm_error_s=[b"SSD/Norm"], transform_2d_s=[b"WHT"], transform_1d_s=[b"Bior1.5"]),
+    dict(bm_error_s=[b"zsad", b"zssd", b"sad"], transform_2d_s=[b"haar", b"dct"]),
+])
+def test_rtc_options_accepted_case_insensitive(kw):
+    # valid options pass on to the next check, the device (no GPU in the CPU
+    # suite; with a GPU the filter is created)
+    rc, got = create(args(device_id=64, **kw), vi())
+    assert got == "BM3D: invalid device ID (64)"
+
+
+def test_rtc_options_after_chroma_check():
+    rc, got = create(args(chroma=1, bm_error_s=[b"bad"]), vi())
+    assert got == 'BM3D: clip format must be YUV444 when "chroma" is true'

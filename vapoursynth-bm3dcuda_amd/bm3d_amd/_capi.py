@@ -33,6 +33,7 @@ class Desc(ctypes.Structure):
         ("ps_num", c_int), ("ps_range", c_int),
         ("chroma", c_int), ("final_", c_int), ("extractor_exp", c_int),
         ("precision", c_int),
+        ("bm_error", c_int), ("transform_2d", c_int), ("transform_1d", c_int),
     ]
 
 
@@ -61,6 +62,9 @@ class Args(ctypes.Structure):
         ("n_extractor_exp", c_int), ("extractor_exp", c_ll),
         ("n_zero_init", c_int), ("zero_init", c_ll),
         ("n_precision", c_int), ("precision", c_ll),
+        ("n_bm_error_s", c_int), ("bm_error_s", ctypes.c_char_p * 3),
+        ("n_transform_2d_s", c_int), ("transform_2d_s", ctypes.c_char_p * 3),
+        ("n_transform_1d_s", c_int), ("transform_1d_s", ctypes.c_char_p * 3),
     ]
 
 

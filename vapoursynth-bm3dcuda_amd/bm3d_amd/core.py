@@ -31,8 +31,26 @@ def clip_desc(t):
     return C.Clip(ctypes.c_void_p(t.data_ptr()), p * h * w, h * w, n)
 
 
+# Options of the NVRTC variant (rtc_source/source.cpp:854-912): names as the
+# plugin accepts them, codes as the C ABI's BM3D_BM_* / BM3D_TRANSFORM_*.
+BM_ERRORS = {"ssd": 0, "sad": 1, "zssd": 2, "zsad": 3, "ssd/norm": 4}
+TRANSFORMS = {"dct": 0, "haar": 1, "wht": 2, "bior1.5": 3}
+
+
+def _option(table, value, name):
+    if isinstance(value, str):
+        key = value.lower()
+        if key not in table:
+            raise ValueError(f"invalid '{name}': {key}")
+        return table[key]
+    if int(value) not in table.values():
+        raise ValueError(f"invalid '{name}': {value}")
+    return int(value)
+
+
 def _desc(width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2, ps_range=4, chroma=False,
-          final=False, extractor_exp=0, precision="fp32", stride=None):
+          final=False, extractor_exp=0, precision="fp32", stride=None, bm_error="ssd", transform_2d="dct",
+          transform_1d="dct"):
     sig = list(sigma) if isinstance(sigma, (list, tuple)) else [sigma]
     sig = (sig + [sig[-1]] * 3)[:3]
     d = C.Desc()
@@ -44,6 +62,9 @@ def _desc(width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2, ps
     d.ps_num, d.ps_range = int(ps_num), int(ps_range)
     d.chroma, d.final_, d.extractor_exp = int(bool(chroma)), int(bool(final)), int(extractor_exp)
     d.precision = {"fp32": C.PRECISION_FP32, "mixed": C.PRECISION_MIXED}[precision]
+    d.bm_error = _option(BM_ERRORS, bm_error, "bm_error_s")
+    d.transform_2d = _option(TRANSFORMS, transform_2d, "transform_2d_s")
+    d.transform_1d = _option(TRANSFORMS, transform_1d, "transform_1d_s")
     return d
 
 
@@ -53,9 +74,9 @@ class Plan:
 
     def __init__(self, width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2,
                  ps_range=4, chroma=False, final=False, extractor_exp=0, precision="fp32",
-                 stride=None):
+                 stride=None, bm_error="ssd", transform_2d="dct", transform_1d="dct"):
         d = _desc(width, height, sigma, block_step, bm_range, radius, ps_num, ps_range, chroma, final,
-                  extractor_exp, precision, stride)
+                  extractor_exp, precision, stride, bm_error, transform_2d, transform_1d)
         self.desc = d
         self.nplanes = 3 if chroma else 1
         self.radius = int(radius)
@@ -199,7 +220,8 @@ class Plan:
 class Batch:
     """Many clips denoised together (bm3d_batch_*): each job is a dict with
     the Plan parameters (width, height, sigma, block_step, bm_range, radius,
-    ps_num, ps_range, chroma, extractor_exp, precision), `src` (a clip
+    ps_num, ps_range, chroma, extractor_exp, precision, bm_error,
+    transform_2d, transform_1d), `src` (a clip
     tensor), `passes` (1: BM3Dv2; 2: two-pass BM3Dv2) and optional `out` /
     `basic` tensors shaped like src (allocated when absent). execute()
     replays the batch's captured graph."""
@@ -219,7 +241,8 @@ class Batch:
             if basic is None and passes == 2:
                 basic = torch.empty_like(src)
             plan_kw = {k: j[k] for k in ("sigma", "block_step", "bm_range", "radius", "ps_num", "ps_range",
-                                         "chroma", "extractor_exp", "precision") if k in j}
+                                         "chroma", "extractor_exp", "precision", "bm_error", "transform_2d",
+                                         "transform_1d") if k in j}
             d = _desc(w, h, **plan_kw)
             arr[i].desc = d
             arr[i].passes = passes

@@ -104,7 +104,7 @@ def _alloc_like(vi, family, subsampling, device):
 
 
 def _args(sigma, block_step, bm_range, radius, ps_num, ps_range, chroma, device_id, fast,
-          extractor_exp, zero_init, precision):
+          extractor_exp, zero_init, precision, bm_error_s=None, transform_2d_s=None, transform_1d_s=None):
     a = C.Args()
 
     def arr(name, vals, conv):
@@ -135,6 +135,9 @@ def _args(sigma, block_step, bm_range, radius, ps_num, ps_range, chroma, device_
     one("zero_init", zero_init)
     if precision is not None:
         one("precision", {"fp32": C.PRECISION_FP32, "mixed": C.PRECISION_MIXED}.get(precision, precision))
+    for name, vals in (("bm_error_s", bm_error_s), ("transform_2d_s", transform_2d_s),
+                       ("transform_1d_s", transform_1d_s)):
+        arr(name, vals, lambda v: v.encode() if isinstance(v, str) else bytes(v))
     return a
 
 
@@ -171,13 +174,16 @@ def _filter(create, clip, ref, kw, stream):
 
 def BM3D(clip, ref=None, sigma=None, block_step=None, bm_range=None, radius=None, ps_num=None,
          ps_range=None, chroma=None, device_id=None, fast=None, extractor_exp=None, zero_init=None,
-         precision=None, stream=None):
+         precision=None, stream=None, bm_error_s=None, transform_2d_s=None, transform_1d_s=None):
     """bm3dcuda.BM3D (source/source.cpp:457-720). With radius > 0 the output
     frames are the stacked weighted sums and weights (height 2(2r+1)H) with
-    props BM3D_V_radius / BM3D_V_process, for VAggregate."""
+    props BM3D_V_radius / BM3D_V_process, for VAggregate. bm_error_s,
+    transform_2d_s and transform_1d_s are the bm3dcuda_rtc options
+    (rtc_source/source.cpp:854-912)."""
     kw = dict(sigma=sigma, block_step=block_step, bm_range=bm_range, radius=radius, ps_num=ps_num,
               ps_range=ps_range, chroma=chroma, device_id=device_id, fast=fast,
-              extractor_exp=extractor_exp, zero_init=zero_init, precision=precision)
+              extractor_exp=extractor_exp, zero_init=zero_init, precision=precision,
+              bm_error_s=bm_error_s, transform_2d_s=transform_2d_s, transform_1d_s=transform_1d_s)
     out, r, process = _filter(C.lib().bm3d_filter_create, clip, ref, kw, stream)
     if r:
         out.props.update(BM3D_V_radius=r, BM3D_V_process=process)
@@ -201,10 +207,13 @@ def VAggregate(clip, src, planes, stream=None):
 
 def BM3Dv2(clip, ref=None, sigma=None, block_step=None, bm_range=None, radius=None, ps_num=None,
            ps_range=None, chroma=None, device_id=None, fast=None, extractor_exp=None,
-           zero_init=None, precision=None, stream=None):
-    """bm3dcuda.BM3Dv2 (source/source.cpp:911-989): BM3D then VAggregate."""
+           zero_init=None, precision=None, stream=None, bm_error_s=None, transform_2d_s=None,
+           transform_1d_s=None):
+    """bm3dcuda.BM3Dv2 (source/source.cpp:911-989): BM3D then VAggregate;
+    with the bm3dcuda_rtc options as BM3D."""
     kw = dict(sigma=sigma, block_step=block_step, bm_range=bm_range, radius=radius, ps_num=ps_num,
               ps_range=ps_range, chroma=chroma, device_id=device_id, fast=fast,
-              extractor_exp=extractor_exp, zero_init=zero_init, precision=precision)
+              extractor_exp=extractor_exp, zero_init=zero_init, precision=precision,
+              bm_error_s=bm_error_s, transform_2d_s=transform_2d_s, transform_1d_s=transform_1d_s)
     out, _, _ = _filter(C.lib().bm3dv2_filter_create, clip, ref, kw, stream)
     return out

@@ -59,6 +59,7 @@ struct Launch {
     int pass = 0;
     int block = 0;  // threads (matching) or waves (filtering)
     int ps_num = 0, final_ = 0, chroma = 0;
+    int variant = 0;  // matching: block distance; filtering: the FP32 generic-transform variant
     size_t lds = 0;
     size_t args_off = 0, grids_off = 0;  // in the table buffer
     int wgs = 0;
@@ -206,17 +207,18 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                 }
             }
             // block matching, grouped by kernel and block size
-            std::map<std::tuple<int, int, int>, std::vector<int>> mgroups;
+            std::map<std::tuple<int, int, int, int>, std::vector<int>> mgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int keys = p->use_dx ? bm3d::dx_temporal_keys(p->d.ps_num) : 0;
-                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby}].push_back(i);
+                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, p->d.bm_error}].push_back(i);
             }
             for (auto& [key, ids] : mgroups) {
                 Launch l{};
                 l.kind = std::get<0>(key) ? 2 : 1;
                 l.pass = pass;
                 l.block = std::get<2>(key);
+                l.variant = std::get<3>(key);
                 std::vector<bm3d::MatchArgs> args;
                 std::vector<bm3d::JobGrid> grids;
                 for (int i : ids) {
@@ -239,12 +241,14 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                 L.push_back(l);
             }
             // filtering, grouped by kernel variant and waves
-            std::map<std::tuple<int, int, int>, std::vector<int>> fgroups;
+            std::map<std::tuple<int, int, int, int>, std::vector<int>> fgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int waves = std::min(p->tune.filter_waves,
                                            p->use_mx_filter ? bm3d::kMaxMxFilterWaves : bm3d::kMaxFilterWaves);
-                fgroups[{p->use_mx_filter ? 1 : 0, p->d.chroma, waves}].push_back(i);
+                // the FP32 filter has a variant for transforms other than the DCT
+                const int generic = !p->use_mx_filter && (p->d.transform_2d || p->d.transform_1d);
+                fgroups[{p->use_mx_filter ? 1 : 0, p->d.chroma, waves, generic}].push_back(i);
             }
             for (auto& [key, ids] : fgroups) {
                 Launch l{};
@@ -253,6 +257,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                 l.chroma = std::get<1>(key);
                 l.final_ = pass;
                 l.block = std::get<2>(key);
+                l.variant = std::get<3>(key);
                 std::vector<bm3d::FilterArgs> args;
                 std::vector<bm3d::JobGrid> grids;
                 for (int i : ids) {
@@ -331,13 +336,13 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                              l.wgs, s);
         case 1:
             return bm3d::launch_match_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
-                                            l.wgs, l.block, l.lds, s);
+                                            l.wgs, l.block, l.lds, l.variant, s);
         case 2:
             return bm3d::launch_match_dx_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
                                                l.wgs, l.block, l.lds, l.ps_num, s);
         case 3:
             return bm3d::launch_filter_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids, n,
-                                             l.wgs, l.final_ != 0, l.chroma != 0, l.block, l.lds, s);
+                                             l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);
         case 4:
             return bm3d::launch_filter_mx_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids,
                                                 n, l.wgs, l.final_ != 0, l.chroma != 0, l.block, l.lds, s);

@@ -166,8 +166,9 @@ TuneCache& tune_cache() {
 std::string tune_key(const bm3d_plan* p) {
     const bm3d_desc& d = p->d;
     char buf[192];
-    std::snprintf(buf, sizeof buf, "w%d h%d s%d bs%d R%d r%d pn%d pr%d c%d f%d p%d", d.width, d.height, d.stride,
-                  d.block_step, d.bm_range, d.radius, d.ps_num, d.ps_range, d.chroma, d.final_, d.precision);
+    std::snprintf(buf, sizeof buf, "w%d h%d s%d bs%d R%d r%d pn%d pr%d c%d f%d p%d e%d t%d%d", d.width, d.height,
+                  d.stride, d.block_step, d.bm_range, d.radius, d.ps_num, d.ps_range, d.chroma, d.final_, d.precision,
+                  d.bm_error, d.transform_2d, d.transform_1d);
     return buf;
 }
 
@@ -256,7 +257,7 @@ bool match_geometry(bm3d_plan* p, int rows, int width) {
 void dx_geometry(bm3d_plan* p) {
     const bm3d_desc& d = p->d;
     p->dx_ok = false;
-    if (d.precision != BM3D_PRECISION_MIXED) return;
+    if (d.precision != BM3D_PRECISION_MIXED || d.bm_error != BM3D_BM_SSD) return;
     const int nWs = 2 * d.bm_range + 1, nWt = 2 * d.ps_range + 1;
     if (nWs * nWs > 4096 || (d.radius > 0 && d.ps_num * nWt * nWt > 4096)) return;
     p->tile_hp2 = ((p->tile_h + 1) / 2 + 1) | 1;
@@ -359,6 +360,7 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.groups = groups;
     m.ncand = ncand;
     m.tile_hp2 = p->tile_hp2;
+    m.cost = d.bm_error;
     return m;
 }
 
@@ -408,6 +410,8 @@ bm3d::FilterArgs filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3
     fa.fx_range = fx_range(p);
     fa.fx_exp = p->fx_exp;
     fa.mx_ops = p->mx_ops;
+    fa.t2d = d.transform_2d;
+    fa.t1d = d.transform_1d;
     return fa;
 }
 
@@ -518,6 +522,10 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         return fail(BM3D_EINVAL, "BM3D: \"extractor_exp\" out of range");
     if (d.precision != BM3D_PRECISION_FP32 && d.precision != BM3D_PRECISION_MIXED)
         return fail(BM3D_EINVAL, "BM3D: unknown precision");
+    if (d.bm_error < 0 || d.bm_error >= bm3d::kNumCosts) return fail(BM3D_EINVAL, "BM3D: unknown bm_error");
+    if (d.transform_2d < 0 || d.transform_2d >= bm3d::kNumTransforms || d.transform_1d < 0 ||
+        d.transform_1d >= bm3d::kNumTransforms)
+        return fail(BM3D_EINVAL, "BM3D: unknown transform");
 
     auto* p = new bm3d_plan;
     p->d = d;
@@ -561,7 +569,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     }
     if (d.precision == BM3D_PRECISION_MIXED) {
         std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);
-        bm3d::build_filter_mx_ops(ops.data());
+        bm3d::build_filter_mx_ops(ops.data(), d.transform_2d, d.transform_1d);
         const size_t bytes = ops.size() * sizeof(uint32_t);
         e = hipMalloc(&p->mx_ops, bytes);
         if (e == hipSuccess) e = hipMemcpy(p->mx_ops, ops.data(), bytes, hipMemcpyHostToDevice);
@@ -722,6 +730,8 @@ int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
         return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision)");
     if ((matching || filter) && p->d.precision != BM3D_PRECISION_MIXED)
         return fail(BM3D_EINVAL, "BM3D: mixed-precision kernels need a plan created with precision MIXED");
+    if (matching && p->d.bm_error != BM3D_BM_SSD)
+        return fail(BM3D_EUNSUPPORTED, "BM3D: the mixed-precision matching kernel computes the ssd distance only");
     if (matching && !p->dx_ok)
         return fail(BM3D_EUNSUPPORTED, "BM3D: search windows too large for the mixed-precision matching kernel");
     p->use_dx = matching != 0;

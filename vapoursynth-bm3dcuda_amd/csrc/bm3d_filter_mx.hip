@@ -21,7 +21,10 @@
 // output layout of one MFMA is the B layout of the next, or the A layout of
 // the transposed matrix), so no lane exchange or LDS staging is needed
 // between the four stages. The operand registers of K, K^T and Dbd are
-// precomputed per lane on the host (build_filter_mx_ops).
+// precomputed per lane on the host (build_filter_mx_ops). The other
+// transforms of the transform_2d_s / transform_1d_s options
+// (rtc_source/kernel.hpp:200-361) replace D and D^T by their forward and
+// inverse matrices in those operands; the kernel is the same.
 //
 // Lane l = 16 q + c. Loads: lane (q, c) reads rows q and q + 4 of block c
 // (member c % 8 of group c / 8). Coefficients: lane (q, c) holds Z[m][u]
@@ -34,6 +37,7 @@
 #include <stdint.h>
 
 #include "bm3d_device.h"
+#include "bm3d_xform.h"
 
 namespace bm3d {
 
@@ -346,11 +350,31 @@ static double dct_entry(int m, int x) {
     return m == 0 ? ::sqrt(2.0) : 2.0 * ::cos((2 * x + 1) * m * pi / 16.0);
 }
 
-void build_filter_mx_ops(uint32_t* out) {
+// Entry (r, k) of the forward (inverse = 0) or inverse matrix of transform
+// `kind`: the DCT pair (inverse = transpose), else bm3d_xform.h.
+static double xform_entry(int kind, int inverse, int r, int k) {
+    if (kind == 0) return inverse ? dct_entry(k, r) : dct_entry(r, k);
+    return kXformMatrices.m[kind - 1][inverse][r * 8 + k];
+}
+
+// With F2, I2 the forward and inverse matrices of transform t2d (rows and
+// columns) and F1, I1 those of t1d (members): K[u][p] = F2[u/8][p/8]
+// F2[u%8][p%8] takes pixels p to coefficients u, Kinv[p][u] = I2[p/8][u/8]
+// I2[p%8][u%8] takes them back, and Fbd, Ibd are F1, I1 per group. The four
+// products are Y^T = X^T K^T, Z^T = Fbd Y^T, W = Z Ibd^T, X' = Kinv W; for
+// the DCT (I = F^T) they are the products of the file comment.
+void build_filter_mx_ops(uint32_t* out, int t2d, int t1d) {
     auto h = [](double v) -> uint16_t { return __builtin_bit_cast(uint16_t, (_Float16)v); };
-    // K[u][p] = D[u / 8][p / 8] D[u % 8][p % 8] for coefficient u and pixel p
-    auto K = [](int u, int p) { return dct_entry(u >> 3, p >> 3) * dct_entry(u & 7, p & 7); };
-    auto Dbd = [](int m, int b) { return (m >> 3) == (b >> 3) ? dct_entry(m & 7, b & 7) : 0.0; };
+    // K[u][p] for coefficient u and pixel p
+    auto K = [&](int u, int p) { return xform_entry(t2d, 0, u >> 3, p >> 3) * xform_entry(t2d, 0, u & 7, p & 7); };
+    // Kinv[p][u]: pixel p from coefficient u
+    auto Kinv = [&](int p, int u) {
+        return xform_entry(t2d, 1, p >> 3, u >> 3) * xform_entry(t2d, 1, p & 7, u & 7);
+    };
+    // member frequency m from member b (forward), member b from m (inverse),
+    // within each of the two groups
+    auto Fbd = [&](int m, int b) { return (m >> 3) == (b >> 3) ? xform_entry(t1d, 0, m & 7, b & 7) : 0.0; };
+    auto Ibd = [&](int b, int m) { return (m >> 3) == (b >> 3) ? xform_entry(t1d, 1, b & 7, m & 7) : 0.0; };
     for (int lane = 0; lane < 64; ++lane) {
         const int q = lane >> 4, c = lane & 15;
         uint16_t e[kMxOpVectors][8];
@@ -358,12 +382,12 @@ void build_filter_mx_ops(uint32_t* out) {
             const int t = v >> 1, s = v & 1;
             // B[k = 8 q + kk][n = c] of Y^T = X^T K^T: pixel (row q + 4 s, column kk), coefficient 16 t + c
             for (int kk = 0; kk < 8; ++kk) e[v][kk] = h(K(16 * t + c, (q + 4 * s) * 8 + kk));
-            // A[r = c][k = 8 q + j] of X' = K^T W: pixel 16 T + c, coefficient 32 s + 16 (j / 4) + 4 q + j % 4
-            for (int j = 0; j < 8; ++j) e[8 + v][j] = h(K(32 * s + 16 * (j >> 2) + 4 * q + (j & 3), 16 * t + c));
+            // A[r = c][k = 8 q + j] of X' = Kinv W: pixel 16 T + c, coefficient 32 s + 16 (j / 4) + 4 q + j % 4
+            for (int j = 0; j < 8; ++j) e[8 + v][j] = h(Kinv(16 * t + c, 32 * s + 16 * (j >> 2) + 4 * q + (j & 3)));
         }
         for (int i = 0; i < 4; ++i) {
-            e[16][i] = h(Dbd(c, 4 * q + i));      // A[m = c][k = 4 q + i]
-            e[16][4 + i] = h(Dbd(4 * q + i, c));  // B[k = 4 q + i][n = c]
+            e[16][i] = h(Fbd(c, 4 * q + i));      // A[m = c][k = 4 q + i]
+            e[16][4 + i] = h(Ibd(c, 4 * q + i));  // B[k = m = 4 q + i][n = b = c]
         }
         for (int v = 0; v < kMxOpVectors; ++v)
             for (int w = 0; w < 4; ++w)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cfloat>
 #include <cmath>
 #include <memory>
@@ -57,6 +58,7 @@ struct bm3d_filter {
     bool process[3]{};
     float sigma[3]{};
     int block_step[3]{}, bm_range[3]{}, ps_num[3]{}, ps_range[3]{};
+    int bm_error[3]{}, transform_2d[3]{}, transform_1d[3]{};  // BM3D_BM_*, BM3D_TRANSFORM_*
     bm3d_plan* plans[3]{};
     int plan_stride[3]{};
     float* scratch = nullptr;
@@ -112,6 +114,10 @@ int get_plan(bm3d_filter* f, int p, int stride, bm3d_plan** out) {
     d.final_ = f->final_ ? 1 : 0;
     d.extractor_exp = f->extractor_exp;
     d.precision = f->precision;
+    // with chroma, plane 0's options serve all planes (rtc_source/source.cpp:987-1000)
+    d.bm_error = f->bm_error[p];
+    d.transform_2d = f->transform_2d[p];
+    d.transform_1d = f->transform_1d[p];
     if (int rc = bm3d_plan_create(&d, &f->plans[p])) return rc;
     f->plan_stride[p] = stride;
     *out = f->plans[p];
@@ -139,6 +145,27 @@ int copy_plane(float* dst, long long dfs, int ds, const float* src, long long sf
 template <typename T, typename V>
 void fill3(T (&dst)[3], int n, const V* vals, T dflt) {
     for (int i = 0; i < 3; ++i) dst[i] = i < n ? (T)vals[i] : (i == 0 ? dflt : dst[i - 1]);
+}
+
+// A string option of the NVRTC variant (rtc_source/source.cpp:854-912):
+// per plane, lower-cased, one of `names` (code = index); a missing entry
+// repeats the previous one and the first defaults to names[0].
+int parse_option(const char* opt, int n, const char* const* vals, const char* const* names, int count,
+                 int (&dst)[3]) {
+    for (int i = 0; i < 3; ++i) {
+        if (i >= n) {
+            dst[i] = i == 0 ? 0 : dst[i - 1];
+            continue;
+        }
+        std::string v = vals[i] ? vals[i] : "";
+        for (char& ch : v) ch = (char)std::tolower((unsigned char)ch);
+        int code = -1;
+        for (int k = 0; k < count; ++k)
+            if (v == names[k]) code = k;
+        if (code < 0) return fail(BM3D_EINVAL, std::string("BM3D: invalid '") + opt + "': " + v);
+        dst[i] = code;
+    }
+    return BM3D_OK;
 }
 
 int create_bm3d(const bm3d_args* a, const bm3d_video_info* vi, const bm3d_video_info* ref, bm3d_filter** out) {
@@ -187,6 +214,19 @@ int create_bm3d(const bm3d_args* a, const bm3d_video_info* vi, const bm3d_video_
     f->chroma = a->n_chroma > 0 && a->chroma != 0;  // 585-595
     if (f->chroma && !is_yuv444ps(*vi))
         return fail(BM3D_EINVAL, "BM3D: clip format must be YUV444 when \"chroma\" is true");
+    {  // the NVRTC variant's options, checked before the device (rtc_source/source.cpp:854-912)
+        static const char* const kErrors[] = {"ssd", "sad", "zssd", "zsad", "ssd/norm"};
+        static const char* const kTransforms[] = {"dct", "haar", "wht", "bior1.5"};
+        if (int rc = parse_option("bm_error_s", std::min(a->n_bm_error_s, 3), a->bm_error_s, kErrors, 5,
+                                  f->bm_error))
+            return rc;
+        if (int rc = parse_option("transform_2d_s", std::min(a->n_transform_2d_s, 3), a->transform_2d_s,
+                                  kTransforms, 4, f->transform_2d))
+            return rc;
+        if (int rc = parse_option("transform_1d_s", std::min(a->n_transform_1d_s, 3), a->transform_1d_s,
+                                  kTransforms, 4, f->transform_1d))
+            return rc;
+    }
     const int dev = a->n_device_id > 0 ? (int)a->device_id : 0;  // 597-611
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;

@@ -29,6 +29,10 @@ struct JobGrid {
     int gx, gy;
 };
 
+// Block distances (BM3D_BM_*) and 8-point transforms (BM3D_TRANSFORM_*).
+constexpr int kNumCosts = 5;
+constexpr int kNumTransforms = 4;
+
 struct MatchArgs {
     const float* clip;       // plane used for matching, frame 0
     long long frame_stride;  // floats between frames
@@ -43,6 +47,7 @@ struct MatchArgs {
     int tile_hp2;            // mixed kernel: FP16 pair words per tile column
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
     unsigned long long* ncand;  // non-null: count in-plane candidates
+    int cost;                // block distance, 0 .. kNumCosts - 1 (the mixed kernel: 0 only)
 };
 
 struct FilterArgs {
@@ -68,6 +73,7 @@ struct FilterArgs {
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
     const void* mx_ops;      // mixed filter: per-lane transform operands (build_filter_mx_ops)
+    int t2d, t1d;            // FP32 filter: transform kinds along rows / columns and members
     // Accumulation mode.
     //  0 (slots): one evaluation per work-group (frame first_frame + z of the
     //    grid); region of slot z of evaluation f at scratch
@@ -168,11 +174,12 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 // (threads: tbw x tby for matching, 64 x waves for filtering) and kernel
 // variant; lds_bytes covers the largest job.
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                              size_t lds_bytes, hipStream_t s);
+                              size_t lds_bytes, int cost, hipStream_t s);
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                  size_t lds_bytes, int ps_num, hipStream_t s);
+// generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
-                               bool chroma, int waves, size_t lds_bytes, hipStream_t s);
+                               bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);
 hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                   bool chroma, int waves, size_t lds_bytes, hipStream_t s);
 hipError_t launch_vagg_targets_multi(const VTargetArgs* jobs, const JobGrid* grids, int njobs, int wgs,
@@ -202,7 +209,9 @@ hipError_t set_filter_lds_limit(size_t bytes);
 // Mixed-precision filter (bm3d_filter_mx.hip): same arguments, tiles and
 // outputs as launch_filter; `waves` waves per work-group (1..kMaxMxFilterWaves).
 constexpr int kMxOpVectors = 17;  // uint4 per lane of FilterArgs::mx_ops
-void build_filter_mx_ops(uint32_t* out);  // kMxOpVectors * 64 * 4 words, host
+// kMxOpVectors * 64 * 4 words, host; transform kinds t2d (rows and columns)
+// and t1d (members).
+void build_filter_mx_ops(uint32_t* out, int t2d, int t1d);
 hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                             hipStream_t s);
 hipError_t set_filter_mx_lds_limit(size_t bytes);

@@ -29,6 +29,7 @@
 #include <stdint.h>
 
 #include "bm3d_scan.h"
+#include "bm3d_xform.h"
 
 namespace bm3d {
 
@@ -98,6 +99,51 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
         dct8_fwd(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c], v[5][c], v[6][c], v[7][c]);
 }
 
+// Transforms other than the DCT (bm3d_xform.h), rounded once to float, in
+// constant memory: a wave-uniform kind reads its matrix through the scalar
+// cache.
+struct XformF32 {
+    float m[3][2][64];
+};
+constexpr XformF32 xform_f32(const XformMatrices& d) {
+    XformF32 f{};
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 64; ++j) f.m[k][i][j] = static_cast<float>(d.m[k][i][j]);
+    return f;
+}
+__constant__ XformF32 kXformF32 = xform_f32(kXformMatrices);
+
+// v <- M v: each output a product with v[0], then an FMA chain over
+// v[1] .. v[7] (zero entries included), as the oracle's xform8.
+__device__ __forceinline__ void xform_mat(float& v0, float& v1, float& v2, float& v3, float& v4, float& v5,
+                                          float& v6, float& v7, const float* __restrict__ M) {
+    const float x[8] = {v0, v1, v2, v3, v4, v5, v6, v7};
+    float o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        float acc = M[r * 8] * x[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) acc = fma_(M[r * 8 + k], x[k], acc);
+        o[r] = acc;
+    }
+    v0 = o[0]; v1 = o[1]; v2 = o[2]; v3 = o[3]; v4 = o[4]; v5 = o[5]; v6 = o[6]; v7 = o[7];
+}
+
+// Transform `kind` (0: the DCT butterflies, else the matrix) of v[0..7] or,
+// strided, of v[0..7][c]. kind is wave-uniform.
+template <bool kInverse>
+__device__ __forceinline__ void xform8(float* v, int kind) {
+    if (kind == 0) dct8<kInverse>(v);
+    else xform_mat(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], kXformF32.m[kind - 1][kInverse]);
+}
+template <bool kInverse>
+__device__ __forceinline__ void xform8_strided(float (&v)[8][8], int c, int kind) {
+    if (kind == 0) dct8_strided<kInverse>(v, c);
+    else xform_mat(v[0][c], v[1][c], v[2][c], v[3][c], v[4][c], v[5][c], v[6][c], v[7][c],
+                   kXformF32.m[kind - 1][kInverse]);
+}
+
 // ---------------------------------------------------------------- matching
 
 // Block matching: one lane per reference block; a work-group covers
@@ -107,7 +153,9 @@ __device__ __forceinline__ void dct8_strided(float (&v)[8][8], int c) {
 // (source/kernel.cu:495-614, cpu_source/source.cpp:220-321), then the centre
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
 // with kCount, also adds the number of in-plane candidates to *a.ncand.
-template <bool kCount>
+// kCost selects the block distance (MatchArgs::cost, the bm_error_s option
+// of rtc_source/kernel.hpp:372-504).
+template <int kCost, bool kCount>
 __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     extern __shared__ float lds[];
     const int wshift = a.tbw == 32 ? 5 : 4;
@@ -155,10 +203,13 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
         }
     }
 
+    BlockStat st{0.0f, 0.0f};
+    if constexpr (kCost >= 2) st = block_stat(rb);
+
     unsigned cnt = 0;
     Best8 best;
     best.init(kFltMax, pack_pos(x, y, r));
-    scan_window<kCount>(lds, g, rb, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+    scan_window<kCost, kCount>(lds, g, rb, st, best, x, y, a.bm_range, r, a.W, a.H, cnt);
 
     if (r > 0) {
         uint32_t centre[8];
@@ -179,7 +230,8 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
                     uint32_t p = last[0];
 #pragma unroll
                     for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                    scan_window<kCount>(lds, g, rb, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H, cnt);
+                    scan_window<kCost, kCount>(lds, g, rb, st, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H,
+                                               cnt);
                 }
                 for (int i = 0; i < a.ps_num; ++i) {
                     const float e = fr.err_at(i);
@@ -213,17 +265,17 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     }
 }
 
-template <bool kCount>
+template <int kCost, bool kCount>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
-    match_body<kCount>(a, launch_wg());
+    match_body<kCost, kCount>(a, launch_wg());
 }
 
-template <bool kCount>
+template <int kCost>
 __global__ __launch_bounds__(256) void bm3d_match_multi(const MatchArgs* __restrict__ jobs,
                                                         const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
-    match_body<kCount>(a, job_wg(grids[j], blockIdx.x));
+    match_body<kCost, false>(a, job_wg(grids[j], blockIdx.x));
 }
 
 // ---------------------------------------------------------------- filtering
@@ -257,27 +309,41 @@ __device__ __forceinline__ void transpose_member(float (&x)[8], int k) {
 
 // Forward 3-D transform: per member along j (column layout), transpose,
 // along k (row layout); then along i. Coefficients end in row layout.
-__device__ __forceinline__ void forward3d(float (&v)[8][8], int k) {
+// kGeneric: transform kinds t2 (rows, columns) and t1 (members) as chosen by
+// the plan (rtc_source/kernel.hpp:585-605); otherwise the DCT throughout.
+template <bool kGeneric>
+__device__ __forceinline__ void forward3d(float (&v)[8][8], int k, int t2, int t1) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        dct8<false>(v[i]);
+        if constexpr (kGeneric) xform8<false>(v[i], t2);
+        else dct8<false>(v[i]);
         transpose_member(v[i], k);
-        dct8<false>(v[i]);
+        if constexpr (kGeneric) xform8<false>(v[i], t2);
+        else dct8<false>(v[i]);
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) dct8_strided<false>(v, c);
+    for (int c = 0; c < 8; ++c) {
+        if constexpr (kGeneric) xform8_strided<false>(v, c, t1);
+        else dct8_strided<false>(v, c);
+    }
 }
 
 // Inverse 3-D transform from row layout: along i; then per member along k,
 // transpose, along j. The estimate ends in column layout.
-__device__ __forceinline__ void inverse3d(float (&v)[8][8], int k) {
+template <bool kGeneric>
+__device__ __forceinline__ void inverse3d(float (&v)[8][8], int k, int t2, int t1) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) dct8_strided<true>(v, c);
+    for (int c = 0; c < 8; ++c) {
+        if constexpr (kGeneric) xform8_strided<true>(v, c, t1);
+        else dct8_strided<true>(v, c);
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        dct8<true>(v[i]);
+        if constexpr (kGeneric) xform8<true>(v[i], t2);
+        else dct8<true>(v[i]);
         transpose_member(v[i], k);
-        dct8<true>(v[i]);
+        if constexpr (kGeneric) xform8<true>(v[i], t2);
+        else dct8<true>(v[i]);
     }
 }
 
@@ -294,7 +360,7 @@ __device__ __forceinline__ void inverse3d(float (&v)[8][8], int k) {
 // are whole multiples of the unit, so the integer sums are exact. Members
 // outside the regions (mode 0 only) go to global float atomics on the
 // stack. Member weights go to a corner map (one atomic per member).
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kGeneric>
 __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
     extern __shared__ float lds[];
     __shared__ SlotTable st;
@@ -355,7 +421,7 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
                 for (int i = 0; i < 8; ++i)
 #pragma unroll
                     for (int j = 0; j < 8; ++j) cf[i][j] = mp[i][roff + (ptrdiff_t)j * a.stride];
-                forward3d(cf, k);
+                forward3d<kGeneric>(cf, k, a.t2d, a.t1d);
                 const float s2 = sigma * sigma;
                 float sum = 0.0f;
 #pragma unroll
@@ -376,7 +442,7 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
                 for (int i = 0; i < 8; ++i)
 #pragma unroll
                     for (int j = 0; j < 8; ++j) v[i][j] = mp[i][(size_t)j * a.stride];
-                forward3d(v, k);
+                forward3d<kGeneric>(v, k, a.t2d, a.t1d);
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -386,7 +452,7 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
                 for (int i = 0; i < 8; ++i)
 #pragma unroll
                     for (int j = 0; j < 8; ++j) v[i][j] = mp[i][(size_t)j * a.stride];
-                forward3d(v, k);
+                forward3d<kGeneric>(v, k, a.t2d, a.t1d);
                 // hard thresholding (source/kernel.cu:208-252)
                 int kept = 0;
 #pragma unroll
@@ -404,7 +470,7 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
                 kept += lane_xor4(kept);
                 w = 1.0f / (float)kept;
             }
-            inverse3d(v, k);
+            inverse3d<kGeneric>(v, k, a.t2d, a.t1d);
 
             if (active) {
                 // aggregation with pre-rounding (source/kernel.cu:717-756)
@@ -448,17 +514,17 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
     });
 }
 
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kGeneric>
 __global__ __launch_bounds__(256) void bm3d_filter(FilterArgs a) {
-    filter_body<kFinal, kChroma>(a, launch_wg());
+    filter_body<kFinal, kChroma, kGeneric>(a, launch_wg());
 }
 
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kGeneric>
 __global__ __launch_bounds__(256) void bm3d_filter_multi(const FilterArgs* __restrict__ jobs,
                                                          const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const FilterArgs& a = jobs[j];
-    filter_body<kFinal, kChroma>(a, job_wg(grids[j], blockIdx.x));
+    filter_body<kFinal, kChroma, kGeneric>(a, job_wg(grids[j], blockIdx.x));
 }
 
 }  // namespace
@@ -742,72 +808,131 @@ __global__ __launch_bounds__(256) void bm3d_vagg_targets_multi(const VTargetArgs
 
 // ---------------------------------------------------------------- launchers
 
+template <int kCost>
+void launch_match_cost(const MatchArgs& a, dim3 grid, dim3 block, size_t lds_bytes, hipStream_t s) {
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match<kCost, true>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match<kCost, false>), grid, block, lds_bytes, s, a);
+}
+
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match<true>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match<false>), grid, block, lds_bytes, s, a);
+    switch (a.cost) {
+        case 0: launch_match_cost<0>(a, grid, block, lds_bytes, s); break;
+        case 1: launch_match_cost<1>(a, grid, block, lds_bytes, s); break;
+        case 2: launch_match_cost<2>(a, grid, block, lds_bytes, s); break;
+        case 3: launch_match_cost<3>(a, grid, block, lds_bytes, s); break;
+        case 4: launch_match_cost<4>(a, grid, block, lds_bytes, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                              size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL((bm3d_match_multi<false>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids, njobs);
+                              size_t lds_bytes, int cost, hipStream_t s) {
+    dim3 grid(wgs), block(threads);
+    switch (cost) {
+        case 0: hipLaunchKernelGGL((bm3d_match_multi<0>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
+        case 1: hipLaunchKernelGGL((bm3d_match_multi<1>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
+        case 2: hipLaunchKernelGGL((bm3d_match_multi<2>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
+        case 3: hipLaunchKernelGGL((bm3d_match_multi<3>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
+        case 4: hipLaunchKernelGGL((bm3d_match_multi<4>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t set_match_lds_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<false>),
-                         reinterpret_cast<const void*>(&bm3d_match<true>),
-                         reinterpret_cast<const void*>(&bm3d_match_multi<false>)};
+template <int kCost>
+hipError_t set_match_cost_lds_limit(size_t bytes) {
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<kCost, false>),
+                         reinterpret_cast<const void*>(&bm3d_match<kCost, true>),
+                         reinterpret_cast<const void*>(&bm3d_match_multi<kCost>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t set_match_lds_limit(size_t bytes) {
+    hipError_t e = set_match_cost_lds_limit<0>(bytes);
+    if (e == hipSuccess) e = set_match_cost_lds_limit<1>(bytes);
+    if (e == hipSuccess) e = set_match_cost_lds_limit<2>(bytes);
+    if (e == hipSuccess) e = set_match_cost_lds_limit<3>(bytes);
+    if (e == hipSuccess) e = set_match_cost_lds_limit<4>(bytes);
+    return e;
+}
+
+template <bool kGeneric>
+void launch_filter_kind(const FilterArgs& a, bool final_, bool chroma, dim3 grid, dim3 block, size_t lds_bytes,
+                        hipStream_t s) {
+    if (final_) {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true, kGeneric>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<true, false, kGeneric>), grid, block, lds_bytes, s, a);
+    } else {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter<false, true, kGeneric>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter<false, false, kGeneric>), grid, block, lds_bytes, s, a);
+    }
 }
 
 hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                          hipStream_t s) {
     dim3 grid(a.ntx, a.nty, grid_z);
     dim3 block(64 * waves);
-    if (final_) {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter<true, true>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_filter<true, false>), grid, block, lds_bytes, s, a);
-    } else {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter<false, true>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_filter<false, false>), grid, block, lds_bytes, s, a);
-    }
+    if (a.t2d < 0 || a.t2d >= kNumTransforms || a.t1d < 0 || a.t1d >= kNumTransforms) return hipErrorInvalidValue;
+    if (a.t2d || a.t1d) launch_filter_kind<true>(a, final_, chroma, grid, block, lds_bytes, s);
+    else launch_filter_kind<false>(a, final_, chroma, grid, block, lds_bytes, s);
     return hipGetLastError();
+}
+
+template <bool kGeneric>
+void launch_filter_multi_kind(const FilterArgs* jobs, const JobGrid* grids, int njobs, bool final_, bool chroma,
+                              dim3 grid, dim3 block, size_t lds_bytes, hipStream_t s) {
+    if (final_) {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_multi<true, true, kGeneric>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_multi<true, false, kGeneric>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+    } else {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_multi<false, true, kGeneric>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_multi<false, false, kGeneric>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+    }
 }
 
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
-                               bool chroma, int waves, size_t lds_bytes, hipStream_t s) {
+                               bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s) {
     dim3 grid(wgs), block(64 * waves);
-    if (final_) {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter_multi<true, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
-        else hipLaunchKernelGGL((bm3d_filter_multi<true, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
-    } else {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter_multi<false, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
-        else hipLaunchKernelGGL((bm3d_filter_multi<false, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
-    }
+    if (generic) launch_filter_multi_kind<true>(jobs, grids, njobs, final_, chroma, grid, block, lds_bytes, s);
+    else launch_filter_multi_kind<false>(jobs, grids, njobs, final_, chroma, grid, block, lds_bytes, s);
     return hipGetLastError();
 }
 
-hipError_t set_filter_lds_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_filter<true, true>),
-                         reinterpret_cast<const void*>(&bm3d_filter<true, false>),
-                         reinterpret_cast<const void*>(&bm3d_filter<false, true>),
-                         reinterpret_cast<const void*>(&bm3d_filter<false, false>),
-                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, true>),
-                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, false>),
-                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, true>),
-                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, false>)};
+template <bool kGeneric>
+hipError_t set_filter_kind_lds_limit(size_t bytes) {
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_filter<true, true, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter<true, false, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter<false, true, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter<false, false, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, true, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<true, false, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, true, kGeneric>),
+                         reinterpret_cast<const void*>(&bm3d_filter_multi<false, false, kGeneric>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t set_filter_lds_limit(size_t bytes) {
+    hipError_t e = set_filter_kind_lds_limit<false>(bytes);
+    if (e == hipSuccess) e = set_filter_kind_lds_limit<true>(bytes);
+    return e;
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s) {

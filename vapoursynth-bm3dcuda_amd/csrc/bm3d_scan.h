@@ -77,9 +77,89 @@ __device__ __forceinline__ float ring_ssd(const float (&rb)[8][8], const float (
     return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
-template <int U, bool kCount>
+__device__ __forceinline__ float tree8(const float (&s)[8]) {
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+// Per-reference-block terms of the distances that need them: the block
+// mean (zssd, zsad) and norm (ssd/norm).
+struct BlockStat {
+    float mean, norm;
+};
+
+// Block mean as rtc_source/kernel.hpp:413-421: per column a pairwise sum of
+// the 8 rows, the column tree, times 1/64. x(c, j) reads column c, row j.
+template <typename F>
+__device__ __forceinline__ float block_mean(F x) {
+    float s[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        s[c] = ((x(c, 0) + x(c, 1)) + (x(c, 2) + x(c, 3))) + ((x(c, 4) + x(c, 5)) + (x(c, 6) + x(c, 7)));
+    return tree8(s) * (1.0f / 64.0f);
+}
+
+// Per column even-row and odd-row product chains (errors[i % 2] of
+// rtc_source/kernel.hpp:372-504, products fused), summed, then the tree.
+template <typename FA, typename FB>
+__device__ __forceinline__ float dot_chain(FA a, FB b) {
+    float s[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float e0 = a(c, 0) * b(c, 0), e1 = a(c, 1) * b(c, 1);
+#pragma unroll
+        for (int j = 2; j < 8; j += 2) {
+            e0 = fma_(a(c, j), b(c, j), e0);
+            e1 = fma_(a(c, j + 1), b(c, j + 1), e1);
+        }
+        s[c] = e0 + e1;
+    }
+    return tree8(s);
+}
+
+__device__ __forceinline__ BlockStat block_stat(const float (&rb)[8][8]) {
+    auto x = [&](int c, int j) { return rb[j][c]; };
+    return {block_mean(x), __builtin_sqrtf(dot_chain(x, x))};
+}
+
+// Distance kCost (BM3D_BM_*) between the reference block rb[row][col] and
+// the candidate in the ring (column c in slot (U + c) & 7), in the order of
+// the oracle's block_cost: ssd (cpu_source/source.cpp:123-136), sad, zssd,
+// zsad, ssd/norm (rtc_source/kernel.hpp:372-504).
+template <int kCost, int U>
+__device__ __forceinline__ float ring_cost(const float (&rb)[8][8], const float (&ring)[8][8], const BlockStat& st) {
+    if constexpr (kCost == 0) {
+        return ring_ssd<U>(rb, ring);
+    } else if constexpr (kCost == 4) {
+        auto r = [&](int c, int j) { return rb[j][c]; };
+        auto n = [&](int c, int j) { return ring[(U + c) & 7][j]; };
+        const float nn = __builtin_sqrtf(dot_chain(n, n));
+        const float dot = dot_chain(r, n);
+        return 2.0f - (2.0f * dot) / fma_(st.norm, nn, kFltEpsilon);
+    } else {
+        float dm = 0.0f;
+        if constexpr (kCost == 2 || kCost == 3)
+            dm = st.mean - block_mean([&](int c, int j) { return ring[(U + c) & 7][j]; });
+        float s[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int sl = (U + c) & 7;
+            float e[2];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float v = rb[j][c] - ring[sl][j];
+                if constexpr (kCost == 2 || kCost == 3) v = v - dm;
+                if constexpr (kCost == 2) e[j & 1] = j < 2 ? v * v : fma_(v, v, e[j & 1]);
+                else e[j & 1] = j < 2 ? fabsf(v) : e[j & 1] + fabsf(v);
+            }
+            s[c] = e[0] + e[1];
+        }
+        return tree8(s);
+    }
+}
+
+template <int kCost, int U, bool kCount>
 __device__ __forceinline__ void scan_step(const float* __restrict__ lds, const TileGeom& g,
-                                          const float (&rb)[8][8], float (&ring)[8][8],
+                                          const float (&rb)[8][8], const BlockStat& st, float (&ring)[8][8],
                                           int& ph, int& colbase, int rt, Best8& best,
                                           int col, int row, bool row_ok, int z, int W,
                                           unsigned& cnt) {
@@ -89,7 +169,7 @@ __device__ __forceinline__ void scan_step(const float* __restrict__ lds, const T
     const bool wrap = ph == g.bs;
     ph = wrap ? 0 : ph;
     colbase += wrap ? (g.hp - (g.bs - 1) * g.wq * g.hp) : g.wq * g.hp;
-    float e = ring_ssd<U>(rb, ring);
+    float e = ring_cost<kCost, U>(rb, ring, st);
     const bool ok = row_ok && col >= 0 && col <= W - 8;
     if constexpr (kCount) cnt += ok ? 1u : 0u;
     if (ok && e < best.e[7]) best.insert(e, pack_pos(col, row, z));
@@ -100,9 +180,9 @@ __device__ __forceinline__ void scan_step(const float* __restrict__ lds, const T
 // the full (2 range + 1)^2 square so the loop is wave-uniform; candidates
 // outside the plane are masked, which leaves the scan order of the valid
 // ones -- and hence tie-breaking -- unchanged.
-template <bool kCount>
+template <int kCost, bool kCount>
 __device__ __forceinline__ void scan_window(const float* __restrict__ lds, const TileGeom& g,
-                                            const float (&rb)[8][8], Best8& best,
+                                            const float (&rb)[8][8], const BlockStat& st, Best8& best,
                                             int cx, int cy, int range, int z, int W, int H,
                                             unsigned& cnt) {
     const int n = 2 * range + 1;
@@ -128,7 +208,8 @@ __device__ __forceinline__ void scan_window(const float* __restrict__ lds, const
         for (int base = 0; base < n; base += 8) {
 #define BM3D_SCAN_STEP(U)                                                                    \
     if (base + U < n)                                                                        \
-        scan_step<U, kCount>(lds, g, rb, ring, ph, colbase, rt, best, col0 + base + U, row, row_ok, z, W, cnt);
+        scan_step<kCost, U, kCount>(lds, g, rb, st, ring, ph, colbase, rt, best, col0 + base + U, row, row_ok, z, \
+                                    W, cnt);
             BM3D_SCAN_STEP(0)
             BM3D_SCAN_STEP(1)
             BM3D_SCAN_STEP(2)
