@@ -34,13 +34,18 @@ frame-parallel over strips on host threads (one per core, at most 16), on a
 bounded sample: one basic + one final BM3D window (radius 1) for 4 frames,
 each split into 8 full-width strips of 135 rows (about 15 s of CPU work).
 
+host workload (--workload host): the same clip in pageable host memory,
+through the plugin-level BM3Dv2 filters' host-frame route (whole-clip calls
+and per-frame calls from 4 threads), beside the same filters on device
+frames. Prints its own JSON line (PCIe-inclusive frames/s).
+
 batch workload (--workload batch): 64 seeded GRAYS clips of 8 frames, each
 drawing its size (360p..2160p), radius (0..2) and sigma (5..25); two-pass
 BM3Dv2 through one captured batch graph, against the same clips run one by
 one through FP32 plans. Prints its own JSON line (aggregate Mpixel/s).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp32]
-                [--workload vbm3d|batch]
+                [--workload vbm3d|batch|host]
 """
 from __future__ import annotations
 
@@ -304,6 +309,140 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_host(args):
+    """The VapourSynth route: the 1080p clip in pageable host memory, through
+    the plugin-level BM3Dv2 filters (bm3d_filter_get_frames_host), two
+    passes. Whole-clip calls (the filter pipelines upload, compute and
+    download in chunks), per-frame calls from 4 threads (as VapourSynth's
+    GetFrame; each call carries its own temporal window), and the same
+    filters on device-resident frames, for comparison. PCIe-inclusive:
+    every input frame crosses to the device and every output frame back."""
+    import ctypes
+    import threading
+
+    import numpy as np
+    import torch
+    from bm3d_amd import _capi as C, core, synth, vs
+
+    torch.cuda.set_device(0)
+    nf = args.frames
+    clean, noisy = synth.make_clip(H, W, nf, 1, noise_sd=15.0, seed=1, motion=True)
+    log(f"generated {nf} frames sha256={synth.sha256(noisy)[:16]}")
+    prec = args.precision
+    if not args.no_autotune:  # fills the process tuning cache the filters' plans read
+        Pipeline(core, torch.from_numpy(noisy).cuda(), prec).autotune()
+    kw = dict(sigma=[PARAMS["sigma"]], block_step=PARAMS["block_step"], bm_range=PARAMS["bm_range"],
+              radius=PARAMS["radius"], ps_num=PARAMS["ps_num"], ps_range=PARAMS["ps_range"], chroma=None,
+              device_id=None, fast=None, extractor_exp=None, zero_init=None, precision=prec)
+    a = vs._args(**kw)
+    host_clip = vs.Clip.from_array(noisy, "GRAY", device="cpu")
+    vi = host_clip.video_info()
+    hb, hf = ctypes.c_void_p(), ctypes.c_void_p()
+    C.check(C.lib().bm3dv2_filter_create(ctypes.byref(a), ctypes.byref(vi), None, ctypes.byref(hb)))
+    C.check(C.lib().bm3dv2_filter_create(ctypes.byref(a), ctypes.byref(vi), ctypes.byref(vi), ctypes.byref(hf)))
+    basic = np.empty_like(noisy)
+    final = np.empty_like(noisy)
+    fr_src = host_clip.host_frames(0, nf - 1)
+    fr_basic = vs.Clip.from_array(torch.from_numpy(basic), "GRAY", device="cpu")
+    basic_t = torch.from_numpy(basic)
+    final_t = torch.from_numpy(final)
+    basic_c = vs.Clip([basic_t[:, 0]], "GRAY", base=basic_t)
+    final_c = vs.Clip([final_t[:, 0]], "GRAY", base=final_t)
+    del fr_basic
+    fb, ff = basic_c.host_frames(0, nf - 1), final_c.host_frames(0, nf - 1)
+
+    def whole():
+        C.check(C.lib().bm3d_filter_get_frames_host(hb, fr_src, None, 0, fb, 0, nf))
+        C.check(C.lib().bm3d_filter_get_frames_host(hf, fr_src, fb, 0, ff, 0, nf))
+
+    def per_frame(threads=4):
+        errs = []
+
+        def run(h, ref, out):
+            def work(t):
+                for n in range(t, nf, threads):
+                    lo, hi = ctypes.c_int(), ctypes.c_int()
+                    C.lib().bm3d_filter_host_range(h, n, 1, ctypes.byref(lo), ctypes.byref(hi))
+                    src = ctypes.cast(ctypes.byref(fr_src, lo.value * ctypes.sizeof(C.HostFrame)),
+                                      ctypes.POINTER(C.HostFrame))
+                    rf = None if ref is None else ctypes.cast(
+                        ctypes.byref(ref, lo.value * ctypes.sizeof(C.HostFrame)), ctypes.POINTER(C.HostFrame))
+                    o = ctypes.cast(ctypes.byref(out, n * ctypes.sizeof(C.HostFrame)), ctypes.POINTER(C.HostFrame))
+                    if C.lib().bm3d_filter_get_frames_host(h, src, rf, lo.value, o, n, 1):
+                        errs.append(C.lib().bm3d_last_error().decode())
+            ts = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+
+        run(hb, None, fb)  # every basic frame before the final pass reads them
+        run(hf, fb, ff)
+        if errs:
+            raise RuntimeError(errs[0])
+
+    def timed(fn, reps):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t) / reps
+
+    t_whole = timed(whole, args.steps)
+    out_whole = final.copy()
+    log(f"host whole-clip: {t_whole * 1e3:.1f} ms per two-pass run")
+    t_pf = timed(per_frame, args.steps)
+    log(f"host per-frame x4 threads: {t_pf * 1e3:.1f} ms per two-pass run")
+    pf_max_abs = float(np.abs(final - out_whole).max())
+    # the same filters on device frames (no PCIe in the loop)
+    src_d = vs.Clip.from_array(noisy, "GRAY")
+    b_d = vs._alloc_like(vi, "GRAY", (0, 0), "cuda")
+    f_d = vs._alloc_like(vi, "GRAY", (0, 0), "cuda")
+    frs, frb, frf = src_d.frames(), b_d.frames(), f_d.frames()
+    stream = core._stream_ptr(torch.cuda.current_stream())
+
+    def device():
+        C.check(C.lib().bm3d_filter_get_frames(hb, ctypes.byref(frs), None, ctypes.byref(frb), 0, nf, stream))
+        C.check(C.lib().bm3d_filter_get_frames(hf, ctypes.byref(frs), ctypes.byref(frb), ctypes.byref(frf), 0, nf,
+                                               stream))
+        torch.cuda.synchronize()
+
+    t_dev = timed(device, args.steps)
+    dev_max_abs = float(np.abs(f_d.numpy() - out_whole).max())
+    C.lib().bm3d_filter_destroy(hb)
+    C.lib().bm3d_filter_destroy(hf)
+    r = PARAMS["radius"]
+    frame_mb = H * W * 4 / 1e6
+    out = {
+        "metric": "1080p frames/s, two-pass BM3Dv2 radius=1 from host frames (PCIe-inclusive)",
+        "value": round(nf / t_whole, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "higher_is_better": True,
+        "dtype": "f32" if prec == "fp32" else "f16+f32",
+        "data": "synthetic (the vbm3d workload's clip, in pageable host memory)",
+        "config": {"workload": "GRAYS 1920x1080 x30 frames, two-pass BM3Dv2 radius=1 through "
+                               "bm3d_filter_get_frames_host", "frames": nf, "precision": prec},
+        "whole_clip_fps": round(nf / t_whole, 3),
+        "per_frame_4_threads_fps": round(nf / t_pf, 3),
+        "device_frames_fps": round(nf / t_dev, 3),
+        "per_frame_max_abs_vs_whole_clip": pf_max_abs,
+        "device_max_abs_vs_host": dev_max_abs,
+        "pcie_mb_per_output_frame": {
+            # basic: clip up, basic down; final: clip and basic up, final down
+            "whole_clip": round(5 * frame_mb, 2),
+            # each call uploads its window of 4r + 1 frames (twice in the final pass)
+            "per_frame": round(((4 * r + 1) * 3 + 2) * frame_mb, 2),
+            # source/source.cpp:316-366: 2r + 1 window slots up (twice in the
+            # final pass) and a 2(2r + 1)-frame stack down, per pass
+            "reference_per_frame": round(((2 * r + 1) * (1 + 2) + 2 * 2 * (2 * r + 1)) * frame_mb, 2),
+        },
+        "psnr_final_db": round(synth.psnr(out_whole, clean), 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,8 +458,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="time only --precision")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the plans' default launch shapes instead of autotuned ones")
-    ap.add_argument("--workload", default="vbm3d", choices=["vbm3d", "batch"],
-                    help="vbm3d: the 1080p two-pass V-BM3D clip (default); batch: the 64-clip mix")
+    ap.add_argument("--workload", default="vbm3d", choices=["vbm3d", "batch", "host"],
+                    help="vbm3d: the 1080p two-pass V-BM3D clip (default); batch: the 64-clip mix; "
+                         "host: the vbm3d clip from host memory through the plugin-level filters")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
 
@@ -339,6 +479,9 @@ def main():
         sys.exit(1)
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "host":
+        run_host(args)
         return
     torch.cuda.set_device(0)
     nf = args.frames

@@ -373,6 +373,42 @@ int bm3d_filter_get_frames(bm3d_filter *filter, const bm3d_frames *clip, const b
                            const bm3d_frames *out, int first, int count, void *stream);
 void bm3d_filter_destroy(bm3d_filter *filter);
 
+/* ---- frames in host memory (the VapourSynth GetFrame route) -------------
+ * Replaces the reference's per-frame staging (source/source.cpp:296-426:
+ * vs_bitblt of every window slot into pinned memory, a graph of HtoD,
+ * kernel and DtoH, a stream sync, vs_bitblt out), with `fast` selecting
+ * kFast = 4 resource sets (source/source.cpp:62, 636-713).
+ *
+ * One frame: plane p's rows at data[p], stride[p] floats apart, in any host
+ * memory (VSFrameRef planes: getReadPtr / getWritePtr). */
+typedef struct bm3d_host_frame {
+    float *data[3];
+    int stride[3];
+} bm3d_host_frame;
+
+/* Input frames lo .. hi that output frames first .. first+count-1 read
+ * (clamped to the clip): BM3D r, BM3Dv2 2r, VAggregate r frames either
+ * side. */
+int bm3d_filter_host_range(const bm3d_filter *filter, int first, int count, int *lo, int *hi);
+
+/* bm3d_filter_get_frames on host frames. clip[i] and ref[i] are frame lo + i
+ * for i = 0 .. hi - lo (bm3d_filter_host_range); ref is the basic estimate
+ * (BM3D, BM3Dv2 final) or the source clip (VAggregate, which reads only
+ * frames first .. first+count-1 of it; other entries may be zero). out[i]
+ * receives output frame first + i. Synchronous: returns once out is
+ * written.
+ *
+ * Each input frame goes to the device once per call, and only the output
+ * frames come back. For BM3Dv2 the per-frame stacks never cross PCIe. Host
+ * rows are staged through pinned buffers by a pool of host threads. Ranges
+ * of more than a few frames run in chunks whose upload, compute and
+ * download overlap on separate streams. Calls from several threads run
+ * concurrently on up to 4 resource sets, or 1 when the filter was created
+ * with fast = 0, each with its own plans and buffers. A call holds its
+ * input range (count + 4r frames) on the device. */
+int bm3d_filter_get_frames_host(bm3d_filter *filter, const bm3d_host_frame *clip, const bm3d_host_frame *ref,
+                                int lo, bm3d_host_frame *out, int first, int count);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *bm3d_last_error(void);
 

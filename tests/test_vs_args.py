@@ -159,3 +159,25 @@ def test_rtc_options_accepted_case_insensitive(kw):
 def test_rtc_options_after_chroma_check():
     rc, got = create(args(chroma=1, bm_error_s=[b"bad"]), vi())
     assert got == 'BM3D: clip format must be YUV444 when "chroma" is true'
+
+
+def test_v2_passthrough_host_frames_need_no_device():
+    """BM3Dv2 with nothing to process returns its source (source/source.cpp:
+    932-942); on host frames that is a host copy, so it runs without a GPU."""
+    import numpy as np
+    from bm3d_amd import vs
+    rng = np.random.default_rng(1)
+    arr = rng.uniform(size=(5, 3, 24, 40)).astype(np.float32)
+    src = vs.Clip.from_array(arr, "YUV", device="cpu")
+    out = vs.BM3Dv2(src, sigma=[0.0], radius=1)
+    assert out.on_host
+    np.testing.assert_array_equal(out.numpy(), arr)
+    lo, hi = ctypes.c_int(), ctypes.c_int()
+    h = ctypes.c_void_p()
+    a = args(sigma=[0.0], radius=2)
+    c = vi(family=3, planes=3, w=40, h=24)
+    assert C.lib().bm3dv2_filter_create(ctypes.byref(a), ctypes.byref(c), None, ctypes.byref(h)) == 0
+    assert C.lib().bm3d_filter_host_range(h, 2, 2, ctypes.byref(lo), ctypes.byref(hi)) == 0
+    assert (lo.value, hi.value) == (2, 3)  # nothing processed: no temporal reach
+    assert C.lib().bm3d_filter_host_range(h, 4, 2, ctypes.byref(lo), ctypes.byref(hi)) == C.BM3D_EINVAL
+    C.lib().bm3d_filter_destroy(h)

@@ -72,6 +72,10 @@ class Frames(ctypes.Structure):
     _fields_ = [("data", c_void_p * 3), ("frame_stride", c_ll * 3), ("stride", c_int * 3)]
 
 
+class HostFrame(ctypes.Structure):
+    _fields_ = [("data", c_void_p * 3), ("stride", c_int * 3)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("match_rows", c_int), ("filter_tile_x", c_int), ("filter_tile_y", c_int),
                 ("filter_waves", c_int), ("batch_frames", c_int), ("chunks", c_int), ("match_width", c_int)]
@@ -177,6 +181,11 @@ def lib():
     L.bm3d_filter_get_frames.restype = c_int
     L.bm3d_filter_destroy.argtypes = [P]
     L.bm3d_filter_destroy.restype = None
+    L.bm3d_filter_host_range.argtypes = [P, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    L.bm3d_filter_host_range.restype = c_int
+    PH = ctypes.POINTER(HostFrame)
+    L.bm3d_filter_get_frames_host.argtypes = [P, PH, PH, c_int, PH, c_int, c_int]
+    L.bm3d_filter_get_frames_host.restype = c_int
     L.bm3d_last_error.argtypes = []
     L.bm3d_last_error.restype = ctypes.c_char_p
     L.bm3d_version.argtypes = []
@@ -203,7 +212,7 @@ EXPORTS = (
     "bm3d_batch_create", "bm3d_batch_execute", "bm3d_batch_get_tuning", "bm3d_batch_set_tuning",
     "bm3d_batch_autotune", "bm3d_batch_info", "bm3d_batch_destroy",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
-    "bm3d_filter_get_frames", "bm3d_filter_destroy",
+    "bm3d_filter_get_frames", "bm3d_filter_destroy", "bm3d_filter_host_range", "bm3d_filter_get_frames_host",
 )
 
 KERNEL_MATCH = 0

@@ -29,9 +29,12 @@ class Error(C.BM3DError):
 
 
 class Clip:
-    """Float32 planes on a HIP device. planes[p] is [frames, H_p, W_p]; when
-    all planes share one size they are views of one [frames, P, H, W] tensor.
-    props holds the clip-level frame properties the filters set."""
+    """Float32 planes on a HIP device, or in host memory (device="cpu": the
+    filters then take the host-frame route, bm3d_filter_get_frames_host, as
+    a VapourSynth plugin whose frames live in host memory would). planes[p]
+    is [frames, H_p, W_p]; when all planes share one size they are views of
+    one [frames, P, H, W] tensor. props holds the clip-level frame
+    properties the filters set."""
 
     def __init__(self, planes, family="GRAY", subsampling=(0, 0), props=None, base=None):
         self.planes = list(planes)
@@ -81,6 +84,10 @@ class Clip:
         vi.num_frames = self.num_frames
         return vi
 
+    @property
+    def on_host(self):
+        return self.planes[0].device.type == "cpu"
+
     def frames(self):
         fr = C.Frames()
         for p, t in enumerate(self.planes[:3]):
@@ -88,6 +95,15 @@ class Clip:
             fr.frame_stride[p] = t.stride(0)
             fr.stride[p] = t.stride(1)
         return fr
+
+    def host_frames(self, lo, hi):
+        """bm3d_host_frame entries of frames lo .. hi (host clips)."""
+        arr = (C.HostFrame * (hi - lo + 1))()
+        for i in range(hi - lo + 1):
+            for p, t in enumerate(self.planes[:3]):
+                arr[i].data[p] = t[lo + i].data_ptr()
+                arr[i].stride[p] = t.stride(1)
+        return arr
 
 
 def _alloc_like(vi, family, subsampling, device):
@@ -152,6 +168,18 @@ def _run(h, clip, second, stream):
     process = (ctypes.c_int * 3)()
     _check(C.lib().bm3d_filter_info(h, ctypes.byref(vi), ctypes.byref(radius), process))
     out = _alloc_like(vi, clip.family, clip.subsampling, clip.planes[0].device)
+    if clip.on_host:
+        # the VapourSynth route: host frames in, host frames out
+        if second is not None and not second.on_host:
+            raise ValueError("clip and ref/src must both be host clips or both device clips")
+        lo, hi = ctypes.c_int(), ctypes.c_int()
+        n = clip.num_frames
+        _check(C.lib().bm3d_filter_host_range(h, 0, n, ctypes.byref(lo), ctypes.byref(hi)))
+        f_in = clip.host_frames(lo.value, hi.value)
+        f_sec = second.host_frames(lo.value, hi.value) if second is not None else None
+        f_out = out.host_frames(0, n - 1)
+        _check(C.lib().bm3d_filter_get_frames_host(h, f_in, f_sec, lo.value, f_out, 0, n))
+        return out, radius.value, list(process)
     f_in = clip.frames()
     f_sec = second.frames() if second is not None else None
     f_out = out.frames()

@@ -32,6 +32,8 @@ int bm3d::fail(int code, const std::string& msg) {
     return code;
 }
 
+void bm3d::clear_error() { g_last_error.clear(); }
+
 namespace {
 
 using bm3d::fail;

@@ -17,10 +17,9 @@
 #include "bm3d_mi355x.h"
 
 using bm3d::fail;
+using bm3d::Kind;
 
 namespace {
-
-enum class Kind { kBM3D, kVAggregate, kBM3Dv2 };
 
 bool same_format(const bm3d_video_info& a, const bm3d_video_info& b) {
     return a.color_family == b.color_family && a.sample_type == b.sample_type &&
@@ -41,40 +40,6 @@ bool is_yuv444ps(const bm3d_video_info& v) {
     } while (0)
 
 }  // namespace
-
-struct bm3d_filter {
-    Kind kind = Kind::kBM3D;
-    bm3d_video_info vi{};      // input clip
-    bm3d_video_info out_vi{};  // output clip
-    int num_frames = 0;
-    // BM3D
-    bool final_ = false;
-    bool chroma = false;
-    bool zero_init = true;
-    int radius = 0;
-    int extractor_exp = 0;
-    int precision = BM3D_PRECISION_FP32;
-    int device = 0;
-    bool process[3]{};
-    float sigma[3]{};
-    int block_step[3]{}, bm_range[3]{}, ps_num[3]{}, ps_range[3]{};
-    int bm_error[3]{}, transform_2d[3]{}, transform_1d[3]{};  // BM3D_BM_*, BM3D_TRANSFORM_*
-    bm3d_plan* plans[3]{};
-    int plan_stride[3]{};
-    float* scratch = nullptr;
-    size_t scratch_cap = 0;
-    // BM3Dv2
-    std::unique_ptr<bm3d_filter> bm3d;  // the BM3D stage
-    std::unique_ptr<bm3d_filter> vagg;  // the VAggregate stage (radius > 0)
-    bool passthrough = false;
-    // VAggregate: planes aggregated, radius from the stack height
-    bool vprocess[3]{};
-
-    ~bm3d_filter() {
-        for (auto* p : plans) bm3d_plan_destroy(p);
-        if (scratch) (void)hipFree(scratch);
-    }
-};
 
 namespace {
 
@@ -234,6 +199,7 @@ int create_bm3d(const bm3d_args* a, const bm3d_video_info* vi, const bm3d_video_
     f->device = dev;
     f->extractor_exp = a->n_extractor_exp > 0 ? (int)a->extractor_exp : 0;  // 623-629
     f->zero_init = a->n_zero_init > 0 ? a->zero_init != 0 : true;          // 631-634
+    f->fast = a->n_fast > 0 ? a->fast != 0 : true;                        // 636-639
     f->precision = a->n_precision > 0 ? (int)a->precision : BM3D_PRECISION_FP32;
     // the reference leaves tiny planes unchecked; the kernels need 8x8
     for (int p = 0; p < vi->num_planes && p < 3; ++p)
@@ -337,6 +303,61 @@ int frames_bm3d(bm3d_filter* f, const bm3d_frames* c, const bm3d_frames* ref, co
     return BM3D_OK;
 }
 
+// BM3Dv2 with radius > 0 as one fused pass per plane group
+// (bm3d_execute_vagg: the BM3D evaluations summed straight into the output
+// frames VAggregate reads, bit for bit the stacks route's values with
+// extractor_exp). Unprocessed planes are the source's (VAggregate copies
+// them, source/source.cpp:838-846). BM3D_EUNSUPPORTED when a plan's tiles
+// cannot hold every group member or the chroma planes are not equally
+// spaced; the caller then takes the stacks route.
+int frames_v2_fused(bm3d_filter* f, const bm3d_frames* c, const bm3d_frames* ref, const bm3d_frames* o, int first,
+                    int count, hipStream_t st) {
+    bm3d_filter* b = f->bm3d.get();
+    BM3D_HIP_F(hipSetDevice(b->device));
+    const int N = f->num_frames;
+    const int np = std::min(f->vi.num_planes, 3);
+    if (b->final_ && !ref) return fail(BM3D_EINVAL, "BM3D: \"ref\" frames missing");
+    auto copy_unprocessed = [&]() -> int {
+        for (int p = 0; p < np; ++p)
+            if (!f->process[p])
+                if (int rc = copy_plane(o->data[p], o->frame_stride[p], o->stride[p], c->data[p], c->frame_stride[p],
+                                        c->stride[p], plane_w(f->vi, p), plane_h(f->vi, p), first, count, st))
+                    return rc;
+        return BM3D_OK;
+    };
+    if (b->chroma) {
+        const long long ps = c->data[1] - c->data[0], ops = o->data[1] - o->data[0];
+        const bool even = c->data[2] - c->data[1] == ps && o->data[2] - o->data[1] == ops &&
+                          c->frame_stride[1] == c->frame_stride[0] && c->frame_stride[2] == c->frame_stride[0] &&
+                          o->frame_stride[1] == o->frame_stride[0] && o->frame_stride[2] == o->frame_stride[0] &&
+                          c->stride[1] == c->stride[0] && c->stride[2] == c->stride[0] && o->stride[1] == o->stride[0] &&
+                          o->stride[2] == o->stride[0] &&
+                          (!ref || (ref->data[1] - ref->data[0] == ps && ref->data[2] - ref->data[1] == ps &&
+                                    ref->frame_stride[0] == c->frame_stride[0]));
+        if (!even) return BM3D_EUNSUPPORTED;
+        bm3d_plan* plan;
+        if (int rc = get_plan(b, 0, c->stride[0], &plan)) return rc;
+        bm3d_clip src{c->data[0], c->frame_stride[0], ps, N};
+        bm3d_clip rc_{ref ? ref->data[0] : nullptr, ref ? ref->frame_stride[0] : 0, ps, N};
+        if (int rc = bm3d_execute_vagg(plan, o->data[0] + (size_t)first * o->frame_stride[0], o->frame_stride[0], ops,
+                                       o->stride[0], &src, b->final_ ? &rc_ : nullptr, first, count, st))
+            return rc;
+        return copy_unprocessed();
+    }
+    for (int p = 0; p < np; ++p) {
+        if (!f->process[p]) continue;
+        bm3d_plan* plan;
+        if (int rc = get_plan(b, p, c->stride[p], &plan)) return rc;
+        bm3d_clip src{c->data[p], c->frame_stride[p], 0, N};
+        bm3d_clip rc_{ref ? ref->data[p] : nullptr, ref ? ref->frame_stride[p] : 0, 0, N};
+        if (ref && ref->frame_stride[p] != c->frame_stride[p]) return BM3D_EUNSUPPORTED;
+        if (int rc = bm3d_execute_vagg(plan, o->data[p] + (size_t)first * o->frame_stride[p], o->frame_stride[p], 0,
+                                       o->stride[p], &src, b->final_ ? &rc_ : nullptr, first, count, st))
+            return rc;
+    }
+    return copy_unprocessed();
+}
+
 // VAggregateGetFrame (source/source.cpp:746-862) for a range of frames.
 // stacks: the BM3D output clip; stack_first: frame index of its data[p].
 int frames_vagg(bm3d_filter* f, const bm3d_frames* stacks, int stack_first, const bm3d_frames* src,
@@ -358,14 +379,54 @@ int frames_vagg(bm3d_filter* f, const bm3d_frames* stacks, int stack_first, cons
     return BM3D_OK;
 }
 
+// Keeps the creation arguments of a filter (strings copied).
+void keep_recipe(bm3d_filter* f, Kind kind, const bm3d_args* a, const bm3d_video_info* clip,
+                 const bm3d_video_info* ref, const long long* planes, int n_planes) {
+    bm3d::FilterRecipe& r = f->recipe;
+    r.kind = kind;
+    if (a) r.args = *a;
+    if (clip) r.clip = *clip;
+    r.has_ref = ref != nullptr;
+    if (ref) r.ref = *ref;
+    if (planes) r.planes.assign(planes, planes + n_planes);
+    if (!a) return;
+    const char* const* src[3] = {a->bm_error_s, a->transform_2d_s, a->transform_1d_s};
+    const int n[3] = {a->n_bm_error_s, a->n_transform_2d_s, a->n_transform_1d_s};
+    r.strings.clear();
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < std::min(n[k], 3); ++i) r.strings.emplace_back(src[k][i] ? src[k][i] : "");
+    const char** dst[3] = {r.args.bm_error_s, r.args.transform_2d_s, r.args.transform_1d_s};
+    size_t at = 0;
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < std::min(n[k], 3); ++i) dst[k][i] = src[k][i] ? r.strings[at++].c_str() : (at++, nullptr);
+}
+
 }  // namespace
+
+namespace bm3d {
+
+int clone_filter(const bm3d_filter* f, bm3d_filter** out) {
+    const FilterRecipe& r = f->recipe;
+    switch (r.kind) {
+        case Kind::kBM3D:
+            return bm3d_filter_create(&r.args, &r.clip, r.has_ref ? &r.ref : nullptr, out);
+        case Kind::kVAggregate:
+            return bm3d_vaggregate_create(r.planes.data(), (int)r.planes.size(), &r.clip, &r.ref, out);
+        default:
+            return bm3dv2_filter_create(&r.args, &r.clip, r.has_ref ? &r.ref : nullptr, out);
+    }
+}
+
+}  // namespace bm3d
 
 extern "C" {
 
 int bm3d_filter_create(const bm3d_args* a, const bm3d_video_info* clip, const bm3d_video_info* ref,
                        bm3d_filter** out) {
     if (!out) return fail(BM3D_EINVAL, "BM3D: null argument");
-    return create_bm3d(a, clip, ref, out);
+    const int rc = create_bm3d(a, clip, ref, out);
+    if (rc == BM3D_OK) keep_recipe(*out, Kind::kBM3D, a, clip, ref, nullptr, 0);
+    return rc;
 }
 
 int bm3d_vaggregate_create(const long long* planes, int n_planes, const bm3d_video_info* clip,
@@ -390,6 +451,7 @@ int bm3d_vaggregate_create(const long long* planes, int n_planes, const bm3d_vid
     }
     int dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) f->device = dev;
+    keep_recipe(f.get(), Kind::kVAggregate, nullptr, clip, src, planes, n_planes);
     *out = f.release();
     return BM3D_OK;
 }
@@ -411,6 +473,7 @@ int bm3dv2_filter_create(const bm3d_args* a, const bm3d_video_info* clip, const 
     f->num_frames = clip->num_frames;
     bool skip = true;
     for (int i = 0; i < std::min(clip->num_planes, 3); ++i) skip &= !process[i];
+    keep_recipe(f.get(), Kind::kBM3Dv2, a, clip, ref, nullptr, 0);
     if (skip) {  // 932-942: the source clip itself
         f->passthrough = true;
         *out = f.release();
@@ -420,6 +483,8 @@ int bm3dv2_filter_create(const bm3d_args* a, const bm3d_video_info* clip, const 
     if (int rc = create_bm3d(a, clip, ref, &b)) return rc;
     f->bm3d.reset(b);
     f->radius = b->radius;
+    f->fast = b->fast;
+    f->device = b->device;
     for (int i = 0; i < 3; ++i) f->process[i] = process[i];
     if (b->radius > 0) {  // 956-977
         long long pl[3];
@@ -466,6 +531,11 @@ int bm3d_filter_get_frames(bm3d_filter* f, const bm3d_frames* clip, const bm3d_f
                 return BM3D_OK;
             }
             if (!f->vagg) return frames_bm3d(f->bm3d.get(), clip, ref, out, first, count, st);
+            {
+                const int rc = frames_v2_fused(f, clip, ref, out, first, count, st);
+                if (rc != BM3D_EUNSUPPORTED) return rc;
+                bm3d::clear_error();
+            }
             // BM3D stacks of the frames the temporal window touches, then VAggregate
             const int r = f->radius;
             const int lo = std::max(first - r, 0), hi = std::min(first + count + r, f->num_frames);

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "bm3d_kernels.h"
 #include "bm3d_mi355x.h"
@@ -13,6 +15,8 @@ namespace bm3d {
 
 // Records `msg` as the calling thread's last error and returns `code`.
 int fail(int code, const std::string& msg);
+// Clears the calling thread's last error (after a handled failure).
+void clear_error();
 
 }  // namespace bm3d
 
@@ -65,5 +69,73 @@ const char* plan_tuning_error(const bm3d_tuning& t);
 MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
 FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
 VTargetArgs plan_vtarget_args(const bm3d_plan* p);
+
+}  // namespace bm3d
+
+namespace bm3d {
+
+enum class Kind { kBM3D, kVAggregate, kBM3Dv2 };
+
+// The arguments a filter was created from, kept so that the host-frame path
+// can create independent copies of the filter (one per resource set).
+struct FilterRecipe {
+    Kind kind = Kind::kBM3D;
+    bm3d_args args{};              // string pointers re-aimed at `strings`
+    std::vector<std::string> strings;
+    bm3d_video_info clip{}, ref{};
+    bool has_ref = false;
+    std::vector<long long> planes;  // VAggregate
+};
+
+// Host-frame state of a filter (bm3d_host.cpp).
+struct HostPool;
+void destroy_host_pool(HostPool* pool);
+
+}  // namespace bm3d
+
+// A plugin-level filter (bm3d_filters.cpp): BM3D, VAggregate or BM3Dv2.
+struct bm3d_filter {
+    bm3d::Kind kind = bm3d::Kind::kBM3D;
+    bm3d_video_info vi{};      // input clip
+    bm3d_video_info out_vi{};  // output clip
+    int num_frames = 0;
+    // BM3D
+    bool final_ = false;
+    bool chroma = false;
+    bool zero_init = true;
+    bool fast = true;          // 4 resource sets for host frames (source/source.cpp:636-639)
+    int radius = 0;
+    int extractor_exp = 0;
+    int precision = BM3D_PRECISION_FP32;
+    int device = 0;
+    bool process[3]{};
+    float sigma[3]{};
+    int block_step[3]{}, bm_range[3]{}, ps_num[3]{}, ps_range[3]{};
+    int bm_error[3]{}, transform_2d[3]{}, transform_1d[3]{};  // BM3D_BM_*, BM3D_TRANSFORM_*
+    bm3d_plan* plans[3]{};
+    int plan_stride[3]{};
+    float* scratch = nullptr;
+    size_t scratch_cap = 0;
+    // BM3Dv2
+    std::unique_ptr<bm3d_filter> bm3d;  // the BM3D stage
+    std::unique_ptr<bm3d_filter> vagg;  // the VAggregate stage (radius > 0)
+    bool passthrough = false;
+    // VAggregate: planes aggregated, radius from the stack height
+    bool vprocess[3]{};
+    // creation arguments and host-frame resource sets
+    bm3d::FilterRecipe recipe;
+    bm3d::HostPool* host = nullptr;
+
+    ~bm3d_filter() {
+        bm3d::destroy_host_pool(host);
+        for (auto* p : plans) bm3d_plan_destroy(p);
+        if (scratch) (void)hipFree(scratch);
+    }
+};
+
+namespace bm3d {
+
+// Creates a filter like `f` from its recipe (same kind, arguments, clips).
+int clone_filter(const bm3d_filter* f, bm3d_filter** out);
 
 }  // namespace bm3d
