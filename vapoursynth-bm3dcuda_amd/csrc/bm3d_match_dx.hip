@@ -50,6 +50,22 @@ constexpr int kKSpatial = 12;  // keys kept in the spatial search (>= 8)
 __device__ __forceinline__ float dot2(f16x2 a, f16x2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
 __device__ __forceinline__ f16x2 as_h2(uint32_t u) { return __builtin_bit_cast(f16x2, u); }
 
+__device__ __forceinline__ void cex(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b);
+    b = max(a, b);
+    a = lo;
+}
+
+// Ascending sort of 8 keys (a 19-comparator network).
+__device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
+    cex(v[0], v[2]); cex(v[1], v[3]); cex(v[4], v[6]); cex(v[5], v[7]);
+    cex(v[0], v[4]); cex(v[1], v[5]); cex(v[2], v[6]); cex(v[3], v[7]);
+    cex(v[0], v[1]); cex(v[2], v[3]); cex(v[4], v[5]); cex(v[6], v[7]);
+    cex(v[2], v[4]); cex(v[3], v[5]);
+    cex(v[1], v[4]); cex(v[3], v[6]);
+    cex(v[1], v[2]); cex(v[3], v[4]); cex(v[5], v[6]);
+}
+
 // The K smallest 32-bit keys, ascending (branch-free min/max cascade).
 template <int K>
 struct KeyList {
@@ -65,6 +81,34 @@ struct KeyList {
             const uint32_t hi = max(k[i], key);
             k[i] = min(k[i], key);
             key = hi;
+        }
+    }
+    // Adds 8 keys at once. For K >= 8: sort them (b), take
+    // c[i] = min(k[i], b[K-1-i]) -- the K smallest of the union, as a
+    // sequence that rises then falls -- and sort c with a bitonic merge,
+    // padded at the front with zeros up to a power of two (constant
+    // comparisons the compiler removes). 11-13 min/max per key instead of
+    // the cascade's 2K.
+    __device__ __forceinline__ void push8(uint32_t (&b)[8]) {
+        if constexpr (K < 8) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) push(b[i]);
+        } else {
+            static_assert(K <= 16, "push8 merges into at most 16 keys");
+            constexpr int P = K <= 8 ? 8 : 16;
+            sort8(b);
+            uint32_t c[P];
+#pragma unroll
+            for (int i = 0; i < P - K; ++i) c[i] = 0u;
+#pragma unroll
+            for (int i = 0; i < K; ++i) c[P - K + i] = (K - 1 - i < 8) ? min(k[i], b[K - 1 - i < 8 ? K - 1 - i : 0]) : k[i];
+#pragma unroll
+            for (int d = P / 2; d >= 1; d /= 2)
+#pragma unroll
+                for (int i = 0; i < P; ++i)
+                    if ((i & d) == 0) cex(c[i], c[i + d]);
+#pragma unroll
+            for (int i = 0; i < K; ++i) k[i] = c[P - K + i];
         }
     }
     // removes and returns the smallest key
@@ -159,13 +203,13 @@ __device__ __forceinline__ void load_pair_col(f16x2 (&dst)[4], const uint32_t* _
 }
 
 // One candidate step: loads the window's new column (word address `addr`),
-// updates the column sums and pushes the candidate's key. Step u of the
-// row is an in-plane candidate iff u - vlo <= vspan (unsigned).
-template <int U, int K, bool kCount>
-__device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const f16x2 (&rp)[8][4], float nr,
-                                        f16x2 (&ring)[8][4], float (&csq)[8], float& N, int addr,
-                                        KeyList<K>& kl, int u, unsigned vlo, unsigned vspan, uint32_t idx,
-                                        unsigned& cnt) {
+// updates the column sums and returns the candidate's key (all ones when it
+// is outside the plane). Step u of the row is an in-plane candidate iff
+// u - vlo <= vspan (unsigned).
+template <int U, bool kCount>
+__device__ __forceinline__ uint32_t dx_step(const uint32_t* __restrict__ lds, const f16x2 (&rp)[8][4], float nr,
+                                            f16x2 (&ring)[8][4], float (&csq)[8], float& N, int addr, int u,
+                                            unsigned vlo, unsigned vspan, uint32_t idx, unsigned& cnt) {
     load_pair_col(ring[(U + 7) & 7], lds + addr);
     csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
     N += csq[(U + 7) & 7];
@@ -186,7 +230,7 @@ __device__ __forceinline__ void dx_step(const uint32_t* __restrict__ lds, const 
     const bool ok = (unsigned)u - vlo <= vspan;
     if constexpr (kCount) cnt += ok ? 1u : 0u;
     const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | (idx & kIdxMask);
-    kl.push(ok ? key : 0xFFFFFFFFu);
+    return ok ? key : 0xFFFFFFFFu;
 }
 
 // Approximate scan of the window of radius `range` around (cx, cy) into kl;
@@ -234,18 +278,20 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
         // whole groups of 8 steps unconditionally (the ring returns to the
         // same registers after 8), then the 0..7 remaining steps
         int base = 0;
-#define BM3D_DX_STEP(U)                                                                              \
-    dx_step<U, K, kCount>(lds, rp, nr, ring, csq, N, ad[U], kl, base + U, vlo, vspan,              \
-                          rowidx + (uint32_t)(base + U), cnt);
+#define BM3D_DX_KEY(U)                                                                               \
+    dx_step<U, kCount>(lds, rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), cnt)
+#define BM3D_DX_STEP(U) kl.push(BM3D_DX_KEY(U));
         for (; base + 8 <= n; base += 8) {
-            BM3D_DX_STEP(0)
-            BM3D_DX_STEP(1)
-            BM3D_DX_STEP(2)
-            BM3D_DX_STEP(3)
-            BM3D_DX_STEP(4)
-            BM3D_DX_STEP(5)
-            BM3D_DX_STEP(6)
-            BM3D_DX_STEP(7)
+            uint32_t keys[8];
+            keys[0] = BM3D_DX_KEY(0);
+            keys[1] = BM3D_DX_KEY(1);
+            keys[2] = BM3D_DX_KEY(2);
+            keys[3] = BM3D_DX_KEY(3);
+            keys[4] = BM3D_DX_KEY(4);
+            keys[5] = BM3D_DX_KEY(5);
+            keys[6] = BM3D_DX_KEY(6);
+            keys[7] = BM3D_DX_KEY(7);
+            kl.push8(keys);
             if (div8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) ad[k] += inc8;
@@ -270,6 +316,7 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
             default: break;
         }
 #undef BM3D_DX_STEP
+#undef BM3D_DX_KEY
     }
 }
 
@@ -526,12 +573,12 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
 }
 
 template <bool kCount, int kKt>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx(MatchArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx(MatchArgs a) {
     match_dx_body<kCount, kKt>(a, launch_wg());
 }
 
 template <int kKt>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void bm3d_match_dx_multi(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
