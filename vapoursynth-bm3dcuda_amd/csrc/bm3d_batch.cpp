@@ -310,7 +310,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     a.dst_plane_stride = j.job.src.plane_stride;
                     a.dst_stride = j.job.desc.stride;
                     a.first_frame = 0;
-                    const int gx = (a.W + 63) / 64, gy = (a.H + 7) / 8;
+                    const int gx = (a.W + 63) / 64, gy = (a.H + bm3d::kVaggRows - 1) / bm3d::kVaggRows;
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy * N * j.nplanes;
                     args.push_back(a);

@@ -132,6 +132,9 @@ struct DivideArgs {
     int nplanes;
 };
 
+// Output rows per work-group of bm3d_vagg_targets (64 columns wide).
+constexpr int kVaggRows = 32;
+
 // VAggregate from target-mode accumulation (bm3d_vagg_targets): output
 // frame n = the regions' integer sum of frame n (times the unit) over the
 // 8 x 8 box sum of its corner weight map. The stacks are never written.

@@ -737,13 +737,16 @@ __device__ __forceinline__ void covering_tiles(int v, int span, float inv_span, 
 // bm3d_den_box); out = num / den. Work-group: 64 x 8 pixels, grid
 // (ceil(W/64), ceil(H/8), count * nplanes).
 __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
-    __shared__ float cm[15][72];      // corner-map window: rows y0-7.., columns x0-7..
-    __shared__ float rowsum[15][64];
+    // A work-group produces 64 x kVaggRows output pixels; thread (tx, ty)
+    // the column x0 + tx of rows ty, ty + 4, ...
+    constexpr int kR = kVaggRows, kRH = kVaggRows + 7;
+    __shared__ float cm[kRH][72];      // corner-map window: rows y0-7.., columns x0-7..
+    __shared__ float rowsum[kRH][64];
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // 0..3
     const int x0 = w.x * 64;
     const int x = x0 + tx;
-    const int y0 = w.y * 8;
+    const int y0 = w.y * kR;
     const int fo = w.z / a.nplanes, p = w.z % a.nplanes;
     if (!((a.plane_mask >> p) & 1u)) return;  // uniform per work-group
     const int n = a.first_frame + fo;
@@ -751,44 +754,66 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
     const int th = a.tby * a.bs, tw_ = a.tbx * a.bs;
     const float fx_unit = ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
     const float* cmap = a.corner + (size_t)slot * a.corner_frame + (size_t)p * a.plane_size;
-    for (int k = threadIdx.x; k < 15 * 71; k += 256) {
+    for (int k = threadIdx.x; k < kRH * 71; k += 256) {
         const int rr = k / 71, cc = k - rr * 71;
         const int yy = y0 - 7 + rr, xx = x0 - 7 + cc;
         cm[rr][cc] = (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) ? cmap[(size_t)yy * a.stride + xx] : 0.0f;
     }
-    // covering tiles and region coordinates of this thread's pixels
+    // covering tiles of this thread's column: xlo and (usually) xlo + 1
     int xlo, xhi;
-    covering_tiles(min(x, a.W - 1), tw_, 1.0f / (float)tw_, a.W - 8, a.rm, a.rw, a.ntx, xlo, xhi);
-    const int lx0 = x - (min(xlo * tw_, a.W - 8) - a.rm);
+    const int xc = min(x, a.W - 1);
+    covering_tiles(xc, tw_, 1.0f / (float)tw_, a.W - 8, a.rm, a.rw, a.ntx, xlo, xhi);
+    const int lxa = xc - (min(xlo * tw_, a.W - 8) - a.rm);
+    const int lxb = xc - (min((xlo + 1) * tw_, a.W - 8) - a.rm);
+    const bool xa = (unsigned)lxa < (unsigned)a.rw;
+    const bool xb = xlo + 1 <= xhi && (unsigned)lxb < (unsigned)a.rw;
     const int region = a.rh * a.rw;
+    const size_t tstride = (size_t)a.nplanes * region;  // ints between consecutive tiles
     const int* scr = a.scratch + (size_t)slot * a.target_stride + (size_t)p * region;
     __syncthreads();
-    for (int rr = ty; rr < 15; rr += 4) {
+    for (int rr = ty; rr < kRH; rr += 4) {
         float s = 0.0f;
 #pragma unroll
         for (int b = 0; b < 8; ++b) s = s + cm[rr][tx + 7 - b];
         rowsum[rr][tx] = s;
     }
     __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
+#pragma unroll 4
+    for (int h = 0; h < kR / 4; ++h) {
         const int jj = ty + 4 * h;
         const int y = y0 + jj;
+        const int yc = min(y, a.H - 1);
         int ylo, yhi;
-        covering_tiles(min(y, a.H - 1), th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo, yhi);
+        covering_tiles(yc, th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo, yhi);
         float dv = 0.0f;
 #pragma unroll
         for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
         long long isum = 0;
-        int ly = min(y, a.H - 1) - (min(ylo * th, a.H - 8) - a.rm);
-        for (int tyy = ylo; tyy <= yhi; ++tyy) {
-            int lx = lx0;
-            for (int txx = xlo; txx <= xhi; ++txx) {
-                const int* reg = scr + ((size_t)tyy * a.ntx + txx) * a.nplanes * region;
-                if ((unsigned)lx < (unsigned)a.rw && (unsigned)ly < (unsigned)a.rh) isum += reg[ly * a.rw + lx];
-                lx -= min((txx + 1) * tw_, a.W - 8) - min(txx * tw_, a.W - 8);
+        if (xhi - xlo <= 1 && yhi - ylo <= 1) {
+            // at most 2 x 2 covering regions: four independent, predicated loads
+            const int lya = yc - (min(ylo * th, a.H - 8) - a.rm);
+            const int lyb = yc - (min((ylo + 1) * th, a.H - 8) - a.rm);
+            const bool ya = (unsigned)lya < (unsigned)a.rh;
+            const bool yb = ylo + 1 <= yhi && (unsigned)lyb < (unsigned)a.rh;
+            const int* r0 = scr + ((size_t)ylo * a.ntx + xlo) * tstride;
+            const int* r1 = r0 + (size_t)a.ntx * tstride;
+            const int v00 = (ya && xa) ? r0[lya * a.rw + lxa] : 0;
+            const int v01 = (ya && xb) ? r0[tstride + lya * a.rw + lxb] : 0;
+            const int v10 = (yb && xa) ? r1[lyb * a.rw + lxa] : 0;
+            const int v11 = (yb && xb) ? r1[tstride + lyb * a.rw + lxb] : 0;
+            isum = ((long long)v00 + v01) + ((long long)v10 + v11);
+        } else {
+            // clamped edge tiles can cover a pixel three times
+            int ly = yc - (min(ylo * th, a.H - 8) - a.rm);
+            for (int tyy = ylo; tyy <= yhi; ++tyy) {
+                int lx = lxa;
+                for (int txx = xlo; txx <= xhi; ++txx) {
+                    const int* reg = scr + ((size_t)tyy * a.ntx + txx) * tstride;
+                    if ((unsigned)lx < (unsigned)a.rw && (unsigned)ly < (unsigned)a.rh) isum += reg[ly * a.rw + lx];
+                    lx -= min((txx + 1) * tw_, a.W - 8) - min(txx * tw_, a.W - 8);
+                }
+                ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
             }
-            ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
         }
         const float num = 0.0f + (float)isum * fx_unit;
         if (y < a.H && x < a.W)
@@ -981,7 +1006,7 @@ hipError_t launch_absmax_multi(const AbsmaxArgs* jobs, const JobGrid* grids, int
 }
 
 hipError_t launch_vagg_targets(const VTargetArgs& a, int count, hipStream_t s) {
-    dim3 grid((a.W + 63) / 64, (a.H + 7) / 8, count * a.nplanes);
+    dim3 grid((a.W + 63) / 64, (a.H + kVaggRows - 1) / kVaggRows, count * a.nplanes);
     hipLaunchKernelGGL(bm3d_vagg_targets, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
