@@ -198,13 +198,24 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
         const float* src = a.src + (size_t)p * a.plane_stride;
         float* res = a.res ? a.res + (size_t)gf * a.res_frame_stride + (size_t)p * a.res_plane_stride : nullptr;
 
-        for (int base = 0; base < ngroups; base += nwaves * 2) {
+        // The group entry of a wave step is loaded one step ahead, so the
+        // member rows it addresses are requested as soon as the step starts
+        // (the compiler cannot move the load above the previous step's
+        // global atomics itself). Prefetching the rows too measured slower.
+        auto entry = [&](int base, bool& act) -> uint32_t {
             const int li = base + wave * 2 + (c >> 3);
             const int bx = tx * a.tbx + li % a.tbx;
             const int by = ty * a.tby + li / a.tbx;
-            const bool active = li < ngroups && bx < a.nbx && by < a.nby;
-            const int blk = active ? by * a.nbx + bx : 0;
-            const uint32_t pk = a.groups[((size_t)gf * a.nblocks + blk) * 8 + (c & 7)];
+            act = li < ngroups && bx < a.nbx && by < a.nby;
+            const int blk = act ? by * a.nbx + bx : 0;
+            return a.groups[((size_t)gf * a.nblocks + blk) * 8 + (c & 7)];
+        };
+        bool next_active = false;
+        uint32_t next_pk = entry(0, next_active);
+        for (int base = 0; base < ngroups; base += nwaves * 2) {
+            const bool active = next_active;
+            const uint32_t pk = next_pk;
+            if (base + nwaves * 2 < ngroups) next_pk = entry(base + nwaves * 2, next_active);
             // positions come from block matching and are in range; the clamps
             // only keep a corrupted list from addressing outside the planes
             const int z = min(pos_z(pk), 2 * a.radius);
