@@ -549,11 +549,16 @@ __device__ __forceinline__ void absmax_body(const float* __restrict__ clip, long
     const int f = w.y / nplanes, p = w.y % nplanes;
     const float* base = clip + (size_t)f * frame_stride + (size_t)p * plane_stride;
     float m = 0.0f;
+    const int bd = blockDim.x;
     for (int y = w.x; y < H; y += rows_step) {
         const float* row = base + (size_t)y * stride;
-        for (int x = threadIdx.x; x < W; x += blockDim.x) {
-            const float v = fabsf(row[x]);
-            m = v <= kFltMax ? fmaxf(m, v) : m;  // skips NaN and inf
+        // four independent loads in flight per thread
+        for (int x = threadIdx.x; x < W; x += 4 * bd) {
+            float v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = x + k * bd < W ? fabsf(row[x + k * bd]) : 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) m = v[k] <= kFltMax ? fmaxf(m, v[k]) : m;  // skips NaN and inf
         }
     }
     m = fmaxf(m, lane_xor1(m));
@@ -562,7 +567,15 @@ __device__ __forceinline__ void absmax_body(const float* __restrict__ clip, long
     m = fmaxf(m, dpp_mov<0x140>(m));
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
-    if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(m));
+    // one atomic per work-group: thousands of waves maxing into one word
+    // serialise at its L2 channel
+    __shared__ float wmax[16];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmaxf(m, wmax[i]);
+        atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(m));
+    }
 }
 
 __global__ __launch_bounds__(256) void bm3d_absmax(const float* __restrict__ clip, long long frame_stride,
