@@ -198,3 +198,28 @@ def test_plugin_filters_with_string_options():
         np.testing.assert_array_equal(final.numpy()[:, p:p + 1], f)
     with pytest.raises(vs.Error, match="invalid 'transform_1d_s': dst"):
         vs.BM3D(src, transform_1d_s="DST")
+
+
+def test_mixed_batch_with_options_equals_plans():
+    """Mixed-precision jobs with different transform pairs share one filter
+    launch of a batch (their MFMA operands travel with each job): bit-equal
+    to each plan's fused path with extractor_exp."""
+    core = _core()
+    H, W, N = 40, 56, 5
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15.0, seed=71, motion=True)
+    src = torch.from_numpy(noisy).cuda()
+    prm = dict(sigma=[15.0], block_step=4, bm_range=4, radius=1, ps_num=2, ps_range=3, extractor_exp=11)
+    jobs, wants = [], []
+    for opts in [(0, 0, 0), (0, 3, 2), (2, 2, 3), (0, 1, 0)]:
+        pb = _plan(core, W, H, prm, False, False, opts, "mixed")
+        pf = _plan(core, W, H, prm, True, False, opts, "mixed")
+        b = pb.execute_vagg(src)
+        wants.append(pf.execute_vagg(src, b).cpu().numpy())
+        jobs.append(dict(src=src, passes=2, precision="mixed", sigma=[15.0], block_step=4, bm_range=4, radius=1,
+                         ps_num=2, ps_range=3, extractor_exp=11, bm_error=opts[0], transform_2d=opts[1],
+                         transform_1d=opts[2]))
+    batch = core.Batch(jobs)
+    outs = batch.execute()
+    torch.cuda.synchronize()
+    for o, w in zip(outs, wants):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
