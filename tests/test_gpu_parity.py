@@ -409,8 +409,8 @@ def test_mixed_groups_agree_fp32_larger(prm):
 
 
 def test_mixed_two_pass_within_005db_of_fp64():
-    """The north-star quality bar: two-pass V-BM3D through the mixed path is
-    within 0.05 dB PSNR of the FP64 oracle running the same pipeline."""
+    """The mixed path's quality bar: two-pass V-BM3D through it is within
+    0.05 dB PSNR of the FP64 oracle running the same pipeline."""
     core = _gpu()
     H, W, N = 96, 128, 4
     clean, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=21, motion=True)

@@ -859,7 +859,7 @@ int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* r
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     unsigned long long* d_cnt = nullptr;
     BM3D_HIP(hipMalloc(&d_cnt, sizeof(unsigned long long)));
-    hipError_t e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st);
+    hipError_t e = bm3d::launch_zero(reinterpret_cast<float*>(d_cnt), 2, 0, 1, st);  // the counter's 64 bits
     if (e == hipSuccess && count > 0) e = launch_matching(p, nullptr, d_cnt, src, ref, first_frame, count, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ncand, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

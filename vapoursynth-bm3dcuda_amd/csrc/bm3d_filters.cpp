@@ -99,7 +99,10 @@ int copy_plane(float* dst, long long dfs, int ds, const float* src, long long sf
             BM3D_HIP_F(hipMemcpy2DAsync(d, (size_t)ds * 4, src + (size_t)n * sfs, (size_t)ss * 4, (size_t)w * 4, h,
                                         hipMemcpyDeviceToDevice, st));
         } else {
-            BM3D_HIP_F(hipMemset2DAsync(d, (size_t)ds * 4, 0, (size_t)w * 4, h, st));
+            // a fill kernel, not hipMemset2DAsync, so that a caller can
+            // capture the call in a graph; at most 65535 rows per launch
+            for (int r0 = 0; r0 < h; r0 += 65535)
+                BM3D_HIP_F(bm3d::launch_zero(d + (size_t)r0 * ds, (size_t)w, (size_t)ds, std::min(65535, h - r0), st));
         }
     }
     return BM3D_OK;

@@ -541,8 +541,9 @@ __global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
 }
 
 // Largest finite |sample| over the planes of `frames` frames: grid (rows,
-// frames * nplanes), 256 threads over the columns; one atomic per wave on
-// the float's bit pattern (non-negative floats order like their bits).
+// frames * nplanes), 256 threads over the columns; one atomic per
+// work-group on the float's bit pattern (non-negative floats order like
+// their bits).
 __device__ __forceinline__ void absmax_body(const float* __restrict__ clip, long long frame_stride,
                                             long long plane_stride, int nplanes, int W, int H, int stride,
                                             float* out, Wg w, int rows_step) {
