@@ -59,7 +59,8 @@ struct Launch {
     int pass = 0;
     int block = 0;  // threads (matching) or waves (filtering)
     int ps_num = 0, final_ = 0, chroma = 0;
-    int variant = 0;  // matching: block distance; filtering: the FP32 generic-transform variant
+    int variant = 0;  // matching: block distance (FP32) or block step divides 8 (mixed);
+                      // filtering: the FP32 generic-transform variant
     size_t lds = 0;
     size_t args_off = 0, grids_off = 0;  // in the table buffer
     int wgs = 0;
@@ -206,12 +207,16 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     L.push_back(l);
                 }
             }
-            // block matching, grouped by kernel and block size
+            // block matching, grouped by kernel variant and block size (the
+            // variant: the distance for the FP32 matcher, the block step's
+            // divisibility for the mixed one)
             std::map<std::tuple<int, int, int, int>, std::vector<int>> mgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int keys = p->use_dx ? bm3d::dx_temporal_keys(p->d.ps_num) : 0;
-                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, p->d.bm_error}].push_back(i);
+                const int variant = p->use_dx ? (bm3d::dx_block_step_divides_8(p->d.block_step) ? 1 : 0)
+                                              : p->d.bm_error;
+                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, variant}].push_back(i);
             }
             for (auto& [key, ids] : mgroups) {
                 Launch l{};
@@ -339,7 +344,7 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                             l.wgs, l.block, l.lds, l.variant, s);
         case 2:
             return bm3d::launch_match_dx_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
-                                               l.wgs, l.block, l.lds, l.ps_num, s);
+                                               l.wgs, l.block, l.lds, l.ps_num, l.variant != 0, s);
         case 3:
             return bm3d::launch_filter_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids, n,
                                              l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);

@@ -178,8 +178,10 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 // variant; lds_bytes covers the largest job.
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, int cost, hipStream_t s);
+// (every job of one launch has the same dx_temporal_keys, and div8 is
+// dx_block_step_divides_8 of each job's block step)
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int ps_num, hipStream_t s);
+                                 size_t lds_bytes, int ps_num, bool div8, hipStream_t s);
 // generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);
@@ -202,6 +204,9 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
 hipError_t set_match_dx_lds_limit(size_t bytes);
 // Keys kept per temporal frame (at least 2 ps_num).
 int dx_temporal_keys(int ps_num);
+// The mixed matcher has one kernel variant for block steps dividing 8 and
+// one for the others.
+bool dx_block_step_divides_8(int block_step);
 // Grid (ntx, nty, grid_z): grid_z = evaluations (mode 0) or chunks (mode 1);
 // `waves` waves per work-group (1..kMaxFilterWaves).
 constexpr int kMaxFilterWaves = 4;

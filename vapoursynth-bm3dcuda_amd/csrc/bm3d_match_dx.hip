@@ -202,15 +202,16 @@ __device__ __forceinline__ void load_pair_col(f16x2 (&dst)[4], const uint32_t* _
     for (int i = 0; i < 4; ++i) dst[i] = as_h2(src[i]);
 }
 
-// One candidate step: loads the window's new column (word address `addr`),
+// One candidate step: loads the window's new column (at `addr`),
 // updates the column sums and returns the candidate's key (all ones when it
 // is outside the plane). Step u of the row is an in-plane candidate iff
-// u - vlo <= vspan (unsigned).
-template <int U, bool kCount>
-__device__ __forceinline__ uint32_t dx_step(const uint32_t* __restrict__ lds, const f16x2 (&rp)[8][4], float nr,
-                                            f16x2 (&ring)[8][4], float (&csq)[8], float& N, int addr, int u,
+// u - vlo <= vspan (unsigned); kCheck false: the caller knows every step
+// is.
+template <int U, bool kCount, bool kCheck>
+__device__ __forceinline__ uint32_t dx_step(const f16x2 (&rp)[8][4], float nr, f16x2 (&ring)[8][4], float (&csq)[8],
+                                            float& N, const uint32_t* __restrict__ addr, int u,
                                             unsigned vlo, unsigned vspan, uint32_t idx, unsigned& cnt) {
-    load_pair_col(ring[(U + 7) & 7], lds + addr);
+    load_pair_col(ring[(U + 7) & 7], addr);
     csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
     N += csq[(U + 7) & 7];
     // half the distance: (|r'|^2 + |c'|^2) / 2 - <r', c'>, rp holding -r'
@@ -227,23 +228,30 @@ __device__ __forceinline__ uint32_t dx_step(const uint32_t* __restrict__ lds, co
     // clamped at zero: a rounded negative would order after every key
     const float approx = fmaxf(d0 + d1, 0.0f);
     N -= csq[U & 7];  // column leaving the window of the next candidate
-    const bool ok = (unsigned)u - vlo <= vspan;
-    if constexpr (kCount) cnt += ok ? 1u : 0u;
     const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | (idx & kIdxMask);
-    return ok ? key : 0xFFFFFFFFu;
+    if constexpr (!kCheck) {
+        if constexpr (kCount) cnt += 1u;
+        return key;
+    } else {
+        const bool ok = (unsigned)u - vlo <= vspan;
+        if constexpr (kCount) cnt += ok ? 1u : 0u;
+        return ok ? key : 0xFFFFFFFFu;
+    }
 }
 
 // Approximate scan of the window of radius `range` around (cx, cy) into kl;
-// scan index of (dyi, dxi) is idx_base + dyi * n + dxi.
-template <int K, bool kCount>
-__device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, const PairGeom& g,
+// scan index of (dyi, dxi) is idx_base + dyi * n + dxi. kCheck false: every
+// candidate of the window is in-plane.
+template <int K, bool kCount, bool kCheck, bool kDiv8>
+__device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const PairGeom& g,
                                           const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
                                           int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
     const int n = 2 * range + 1;
-    // with block_step dividing 8, 8 columns on are the same phase 8 / bs
-    // slots further: the step addresses of a row advance by a constant
-    const bool div8 = (8 % g.bs) == 0;
-    const int inc8 = div8 ? (8 / g.bs) * g.hp2 : 0;
+    // kDiv8 (block_step divides 8): 8 columns on are the same phase 8 / bs
+    // slots further, so the step addresses of a row advance by a constant
+    // (a compile-time choice: a branch here would split the loop body and
+    // cost register copies of every column loaded)
+    const int inc8 = kDiv8 ? (8 / g.bs) * g.hp2 : 0;
     f16x2 ring[8][4];
     float csq[8];
     for (int dyi = 0; dyi < n; ++dyi) {
@@ -262,10 +270,10 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
             csq[s] = col_sq(ring[s]);
             N += csq[s];
         }
-        int ad[8];  // word addresses of the new column of steps base .. base + 7
+        const uint32_t* ad[8];  // the new column of steps base .. base + 7
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            ad[k] = off + colbase;
+            ad[k] = lds + off + colbase;
             next_pair_col(g, ph, colbase);
         }
         // in-plane steps: row inside, column col0 + u in [0, W - 8]
@@ -279,7 +287,7 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
         // same registers after 8), then the 0..7 remaining steps
         int base = 0;
 #define BM3D_DX_KEY(U)                                                                               \
-    dx_step<U, kCount>(lds, rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), cnt)
+    dx_step<U, kCount, kCheck>(rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), cnt)
 #define BM3D_DX_STEP(U) kl.push(BM3D_DX_KEY(U));
         for (; base + 8 <= n; base += 8) {
             uint32_t keys[8];
@@ -292,13 +300,13 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
             keys[6] = BM3D_DX_KEY(6);
             keys[7] = BM3D_DX_KEY(7);
             kl.push8(keys);
-            if (div8) {
+            if constexpr (kDiv8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) ad[k] += inc8;
             } else {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    ad[k] = off + colbase;
+                    ad[k] = lds + off + colbase;
                     next_pair_col(g, ph, colbase);
                 }
             }
@@ -318,6 +326,19 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
 #undef BM3D_DX_STEP
 #undef BM3D_DX_KEY
     }
+}
+
+// dx_rows, without the in-plane checks when every lane's window lies inside
+// the plane.
+template <int K, bool kCount, bool kDiv8>
+__device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, const PairGeom& g,
+                                          const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
+                                          int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
+    const bool inside = cx - range >= 0 && cx + range <= W - 8 && cy - range >= 0 && cy + range <= H - 8;
+    if (__all(inside))
+        dx_rows<K, kCount, false, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
+    else
+        dx_rows<K, kCount, true, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
 }
 
 // Exact distance between the block at (x, y) of `rplane` and the candidate
@@ -399,7 +420,7 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
 // One lane per reference block; a work-group covers TBW x TBY blocks (TBW =
 // a.tbw, 16 or 32; TBY = blockDim.x / TBW); grid (ceil(nbx / TBW),
 // ceil(nby / TBY), frames). kKt: keys kept per temporal frame (>= 2 ps_num).
-template <bool kCount, int kKt>
+template <bool kCount, int kKt, bool kDiv8>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
@@ -461,7 +482,7 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
             const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
             KeyList<kKSpatial> kl;
             kl.init();
-            dx_window<kKSpatial, kCount>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
+            dx_window<kKSpatial, kCount, kDiv8>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
             Best8Id bi;
             bi.init(kFltMax, pack_pos(x, y, r));
             const int nW = 2 * a.bm_range + 1;
@@ -509,7 +530,7 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
                         uint32_t p = last[0];
 #pragma unroll
                         for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                        dx_window<kKt, kCount>(lds, pg, rp, nr, kl, pos_x(p), pos_y(p), a.ps_range, a.W, a.H,
+                        dx_window<kKt, kCount, kDiv8>(lds, pg, rp, nr, kl, pos_x(p), pos_y(p), a.ps_range, a.W, a.H,
                                                (uint32_t)(i * nWt * nWt), cnt);
                     }
                     Best8Id fi;
@@ -572,70 +593,85 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     }
 }
 
-template <bool kCount, int kKt>
+template <bool kCount, int kKt, bool kDiv8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx(MatchArgs a) {
-    match_dx_body<kCount, kKt>(a, launch_wg());
+    match_dx_body<kCount, kKt, kDiv8>(a, launch_wg());
 }
 
-template <int kKt>
+template <int kKt, bool kDiv8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, kKt>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, kKt, kDiv8>(a, job_wg(grids[j], blockIdx.x));
 }
 
 namespace {
-template <int kKt>
+template <int kKt, bool kDiv8>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt>), grid, block, lds_bytes, s, a);
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kDiv8>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kDiv8>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
-template <int kKt>
+template <int kKt, bool kDiv8>
 hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                            size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids, njobs);
+    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt, kDiv8>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids,
+                       njobs);
     return hipGetLastError();
 }
-template <int kKt>
+template <int kKt, bool kDiv8>
 hipError_t set_dx_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kDiv8>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kDiv8>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kDiv8>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
+template <bool kDiv8>
+hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    switch (dx_temporal_keys(a.ps_num)) {
+        case 4: return launch_dx<4, kDiv8>(a, tby, lds_bytes, frames, s);
+        case 8: return launch_dx<8, kDiv8>(a, tby, lds_bytes, frames, s);
+        default: return launch_dx<16, kDiv8>(a, tby, lds_bytes, frames, s);
+    }
+}
+template <bool kDiv8>
+hipError_t launch_dx_multi_keys(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
+                                size_t lds_bytes, int ps_num, hipStream_t s) {
+    switch (dx_temporal_keys(ps_num)) {
+        case 4: return launch_dx_multi<4, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        case 8: return launch_dx_multi<8, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        default: return launch_dx_multi<16, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+    }
+}
 }  // namespace
 
 int dx_temporal_keys(int ps_num) { return ps_num <= 2 ? 4 : (ps_num <= 4 ? 8 : 16); }
 
+bool dx_block_step_divides_8(int block_step) { return 8 % block_step == 0; }
+
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
-    switch (dx_temporal_keys(a.ps_num)) {
-        case 4: return launch_dx<4>(a, tby, lds_bytes, frames, s);
-        case 8: return launch_dx<8>(a, tby, lds_bytes, frames, s);
-        default: return launch_dx<16>(a, tby, lds_bytes, frames, s);
-    }
+    return dx_block_step_divides_8(a.bs) ? launch_dx_keys<true>(a, tby, lds_bytes, frames, s)
+                                         : launch_dx_keys<false>(a, tby, lds_bytes, frames, s);
 }
 
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int ps_num, hipStream_t s) {
-    switch (dx_temporal_keys(ps_num)) {
-        case 4: return launch_dx_multi<4>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
-        case 8: return launch_dx_multi<8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
-        default: return launch_dx_multi<16>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
-    }
+                                 size_t lds_bytes, int ps_num, bool div8, hipStream_t s) {
+    return div8 ? launch_dx_multi_keys<true>(jobs, grids, njobs, wgs, threads, lds_bytes, ps_num, s)
+                : launch_dx_multi_keys<false>(jobs, grids, njobs, wgs, threads, lds_bytes, ps_num, s);
 }
 
 hipError_t set_match_dx_lds_limit(size_t bytes) {
-    hipError_t e = set_dx_limit<4>(bytes);
-    if (e == hipSuccess) e = set_dx_limit<8>(bytes);
-    if (e == hipSuccess) e = set_dx_limit<16>(bytes);
+    hipError_t e = hipSuccess;
+    for (hipError_t (*f)(size_t) : {set_dx_limit<4, true>, set_dx_limit<8, true>, set_dx_limit<16, true>,
+                                    set_dx_limit<4, false>, set_dx_limit<8, false>, set_dx_limit<16, false>})
+        if (e == hipSuccess) e = f(bytes);
     return e;
 }
 
