@@ -210,25 +210,26 @@ __device__ __forceinline__ void load_pair_col(f16x2 (&dst)[4], const uint32_t* _
 template <int U, bool kCount, bool kCheck>
 __device__ __forceinline__ uint32_t dx_step(const f16x2 (&rp)[8][4], float nr, f16x2 (&ring)[8][4], float (&csq)[8],
                                             float& N, const uint32_t* __restrict__ addr, int u,
-                                            unsigned vlo, unsigned vspan, uint32_t idx, unsigned& cnt) {
+                                            unsigned vlo, unsigned vspan, uint32_t idx, uint32_t hi_mask,
+                                            unsigned& cnt) {
     load_pair_col(ring[(U + 7) & 7], addr);
     csq[(U + 7) & 7] = col_sq(ring[(U + 7) & 7]);
     N += csq[(U + 7) & 7];
-    // half the distance: (|r'|^2 + |c'|^2) / 2 - <r', c'>, rp holding -r'
-    float d0 = fma_(N, 0.5f, nr), d1 = 0.0f;
+    // half the distance: (|r'|^2 + |c'|^2) / 2 - <r', c'>, rp holding -r',
+    // in one chain
+    float d = fma_(N, 0.5f, nr);
 #pragma unroll
-    for (int c = 0; c < 8; c += 2) {
-        const int s0 = (U + c) & 7, s1 = (U + c + 1) & 7;
+    for (int c = 0; c < 8; ++c) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            d0 = dot2(rp[c][i], ring[s0][i], d0);
-            d1 = dot2(rp[c + 1][i], ring[s1][i], d1);
-        }
+        for (int i = 0; i < 4; ++i) d = dot2(rp[c][i], ring[(U + c) & 7][i], d);
     }
     // clamped at zero: a rounded negative would order after every key
-    const float approx = fmaxf(d0 + d1, 0.0f);
+    const float approx = fmaxf(d, 0.0f);
     N -= csq[U & 7];  // column leaving the window of the next candidate
-    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & ~kIdxMask) | (idx & kIdxMask);
+    // idx < 2^kIdxBits (the plan allows at most 4096 candidates per list);
+    // hi_mask, ~kIdxMask in a register, makes this one and-or with the
+    // wave-uniform index as its scalar operand
+    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & hi_mask) | idx;
     if constexpr (!kCheck) {
         if constexpr (kCount) cnt += 1u;
         return key;
@@ -252,6 +253,8 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
     // (a compile-time choice: a branch here would split the loop body and
     // cost register copies of every column loaded)
     const int inc8 = kDiv8 ? (8 / g.bs) * g.hp2 : 0;
+    uint32_t hi_mask;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(hi_mask) : "i"(~kIdxMask));
     f16x2 ring[8][4];
     float csq[8];
     for (int dyi = 0; dyi < n; ++dyi) {
@@ -287,7 +290,8 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
         // same registers after 8), then the 0..7 remaining steps
         int base = 0;
 #define BM3D_DX_KEY(U)                                                                               \
-    dx_step<U, kCount, kCheck>(rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), cnt)
+    dx_step<U, kCount, kCheck>(rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), \
+                               hi_mask, cnt)
 #define BM3D_DX_STEP(U) kl.push(BM3D_DX_KEY(U));
         for (; base + 8 <= n; base += 8) {
             uint32_t keys[8];
