@@ -9,8 +9,8 @@ bitonic merge padded at the front with zeros (KeyList::push8).
   0-1 principle: a comparator network sorts every input iff it sorts all
   2^8 inputs of zeros and ones.
 * The merge is restated step by step and checked against sorted(keys)[:K]
-  for every K the kernel allows (8..16), with random and heavily tied keys,
-  over whole windows of 8-key batches.
+  for every K the kernel allows (4 and 8..16), with random and heavily tied
+  keys, over whole windows of 8-key batches.
 """
 import os
 import random
@@ -52,9 +52,28 @@ def test_sort8_network_sorts_every_input():
         assert _apply(net, bits) == sorted(bits), bits
 
 
+def _cex_all(v, pairs):
+    for a, b in pairs:
+        if v[a] > v[b]:
+            v[a], v[b] = v[b], v[a]
+    return v
+
+
+def _push8_k4(k, b):
+    """KeyList<4>::push8: the 4 smallest of b by sorted halves and the
+    min(lo[i], hi[3-i]) pairing, then the same pairing into k."""
+    sort4 = [(0, 1), (2, 3), (0, 2), (1, 3), (1, 2)]
+    bitonic4 = [(0, 2), (1, 3), (0, 1), (2, 3)]
+    b = _cex_all(list(b[:4]), sort4) + _cex_all(list(b[4:]), sort4)
+    c = _cex_all([min(b[i], b[7 - i]) for i in range(4)], bitonic4)
+    return _cex_all([min(k[i], c[3 - i]) for i in range(4)], bitonic4)
+
+
 def _push8(k, b, net):
     """KeyList<K>::push8 for K >= 8: returns the new ascending list."""
     K = len(k)
+    if K == 4:
+        return _push8_k4(k, b)
     P = 8 if K <= 8 else 16
     b = _apply(net, b)
     c = [0] * (P - K) + [min(k[i], b[K - 1 - i]) if K - 1 - i < 8 else k[i] for i in range(K)]
@@ -67,7 +86,7 @@ def _push8(k, b, net):
     return c[P - K:]
 
 
-@pytest.mark.parametrize("K", list(range(8, 17)))
+@pytest.mark.parametrize("K", [4] + list(range(8, 17)))
 @pytest.mark.parametrize("spread", [1 << 32, 40, 3])
 def test_push8_keeps_the_k_smallest(K, spread):
     net = _sort8_network()

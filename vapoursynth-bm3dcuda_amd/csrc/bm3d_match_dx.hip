@@ -91,8 +91,22 @@ struct KeyList {
     // padded at the front with zeros up to a power of two (constant
     // comparisons the compiler removes). 11-13 min/max per key instead of
     // the cascade's 2K.
+    // K = 4: the 4 smallest of b come from sorting each half and taking
+    // min(lo[i], hi[3-i]) (a rise-then-fall sequence, sorted by a 4-entry
+    // bitonic merge); the same pairing then merges them into k. 44 min/max
+    // for 8 keys instead of the cascade's 64.
     __device__ __forceinline__ void push8(uint32_t (&b)[8]) {
-        if constexpr (K < 8) {
+        if constexpr (K == 4) {
+            cex(b[0], b[1]); cex(b[2], b[3]); cex(b[0], b[2]); cex(b[1], b[3]); cex(b[1], b[2]);
+            cex(b[4], b[5]); cex(b[6], b[7]); cex(b[4], b[6]); cex(b[5], b[7]); cex(b[5], b[6]);
+            uint32_t c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = min(b[i], b[7 - i]);
+            cex(c[0], c[2]); cex(c[1], c[3]); cex(c[0], c[1]); cex(c[2], c[3]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) k[i] = min(k[i], c[3 - i]);
+            cex(k[0], k[2]); cex(k[1], k[3]); cex(k[0], k[1]); cex(k[2], k[3]);
+        } else if constexpr (K < 8) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) push(b[i]);
         } else {
