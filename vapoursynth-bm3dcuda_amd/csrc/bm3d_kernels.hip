@@ -756,6 +756,7 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
     constexpr int kR = kVaggRows, kRH = kVaggRows + 7;
     __shared__ float cm[kRH][72];      // corner-map window: rows y0-7.., columns x0-7..
     __shared__ float rowsum[kRH][64];
+    __shared__ int2 ycover[kR];  // covering tile rows [lo, hi] of each output row
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // 0..3
     const int x0 = w.x * 64;
@@ -784,6 +785,12 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
     const int region = a.rh * a.rw;
     const size_t tstride = (size_t)a.nplanes * region;  // ints between consecutive tiles
     const int* scr = a.scratch + (size_t)slot * a.target_stride + (size_t)p * region;
+    if (threadIdx.x < kR) {  // once per row, not once per lane
+        int ylo, yhi;
+        covering_tiles(min(y0 + (int)threadIdx.x, a.H - 1), th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo,
+                       yhi);
+        ycover[threadIdx.x] = make_int2(ylo, yhi);
+    }
     __syncthreads();
     for (int rr = ty; rr < kRH; rr += 4) {
         float s = 0.0f;
@@ -797,8 +804,8 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
         const int jj = ty + 4 * h;
         const int y = y0 + jj;
         const int yc = min(y, a.H - 1);
-        int ylo, yhi;
-        covering_tiles(yc, th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo, yhi);
+        const int2 yr = ycover[jj];
+        const int ylo = yr.x, yhi = yr.y;
         float dv = 0.0f;
 #pragma unroll
         for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
