@@ -594,6 +594,9 @@ void bm3d_plan_destroy(bm3d_plan* p) {
     if (p->scratch) (void)hipFree(p->scratch);
     if (p->mx_ops) (void)hipFree(p->mx_ops);
     if (p->range) (void)hipFree(p->range);
+    if (p->side) (void)hipStreamDestroy(p->side);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    if (p->join) (void)hipEventDestroy(p->join);
     delete p;
 }
 
@@ -934,10 +937,22 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     va.dst_plane_stride = dst_plane_stride;
     va.dst_stride = dst_stride;
 
-    BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), stream));
+    // the input range and the first launch's zero fills run on the side
+    // stream, beside the first matching launch (which needs neither); the
+    // first filtering launch waits for them
+    if (!p->side) {
+        BM3D_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        BM3D_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+        BM3D_HIP(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
+    }
+    BM3D_HIP(hipEventRecord(p->fork, stream));
+    BM3D_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+    BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), p->side));
     int next_out = first_frame;
     int touched_hi = tlo_all - 1;  // highest output frame a previous launch added to
     for (int b0 = mlo; b0 <= mhi;) {
+        const bool first_launch = b0 == mlo;
+        hipStream_t zs = first_launch ? p->side : stream;
         const int b1 = std::min(b0 + batch, mhi + 1);
         const int nf = b1 - b0;
         const int chunk = (nf + p->tune.chunks - 1) / std::max(1, p->tune.chunks);
@@ -951,7 +966,7 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
             const int slot = t % ring;
             const int run = std::min(tnew_hi - t + 1, ring - slot);
             BM3D_HIP(bm3d::launch_zero(p->corner + (size_t)slot * corner_target, (size_t)run * corner_target, 0, 1,
-                                       stream));
+                                       zs));
             t += run;
         }
         for (int t = tnew_lo; t <= tnew_hi; ++t) {
@@ -960,13 +975,15 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
             const bool whole = clo >= b0 && chi < b1 && chi < b0 + (k + 1) * chunk;
             if (!whole)
                 BM3D_HIP(bm3d::launch_zero(reinterpret_cast<float*>(p->scratch) + (size_t)(t % ring) * p->region_ints,
-                                           (size_t)p->region_ints, 0, 1, stream));
+                                           (size_t)p->region_ints, 0, 1, zs));
         }
         touched_hi = std::max(touched_hi, tnew_hi);
+        if (first_launch) BM3D_HIP(hipEventRecord(p->join, p->side));
         {
             ProfScope ps(0, stream);
             BM3D_HIP(launch_matching(p, p->groups, nullptr, src, ref, b0, nf, stream));
         }
+        if (first_launch) BM3D_HIP(hipStreamWaitEvent(stream, p->join, 0));
         fa.eval_lo = b0;
         fa.eval_hi = b1;
         fa.chunk = chunk;

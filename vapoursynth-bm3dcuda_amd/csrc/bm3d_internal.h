@@ -56,6 +56,11 @@ struct bm3d_plan {
     // mixed-precision filter kernel (precision MIXED): per-lane transform operands
     bool use_mx_filter = false;
     void* mx_ops = nullptr;
+    // bm3d_execute_vagg: a second stream for the work matching does not wait
+    // for (input range, zero fills), forked from and joined back into the
+    // caller's stream, so that it overlaps the first matching launch
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace bm3d {
