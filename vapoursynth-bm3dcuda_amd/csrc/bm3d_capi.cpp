@@ -571,6 +571,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         bm3d_plan_destroy(p);
         return hip_fail(e, "hipMalloc");
     }
+
     if (d.precision == BM3D_PRECISION_MIXED) {
         std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);
         bm3d::build_filter_mx_ops(ops.data(), d.transform_2d, d.transform_1d);
@@ -940,19 +941,29 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     // the input range and the first launch's zero fills run on the side
     // stream, beside the first matching launch (which needs neither); the
     // first filtering launch waits for them
+    // (created on first use outside a graph capture, since batches never
+    // need it; a call captured before that runs on one stream)
     if (!p->side) {
-        BM3D_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-        BM3D_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
-        BM3D_HIP(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        BM3D_HIP(hipStreamIsCapturing(stream, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            BM3D_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+            BM3D_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+            BM3D_HIP(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
+        }
     }
-    BM3D_HIP(hipEventRecord(p->fork, stream));
-    BM3D_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
-    BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), p->side));
+    hipStream_t side = stream;
+    if (p->side && p->fork && p->join) {
+        side = p->side;
+        BM3D_HIP(hipEventRecord(p->fork, stream));
+        BM3D_HIP(hipStreamWaitEvent(side, p->fork, 0));
+    }
+    BM3D_HIP(measure_range(p, src, std::max(mlo - r, 0), std::min(mhi + r, N - 1), side));
     int next_out = first_frame;
     int touched_hi = tlo_all - 1;  // highest output frame a previous launch added to
     for (int b0 = mlo; b0 <= mhi;) {
         const bool first_launch = b0 == mlo;
-        hipStream_t zs = first_launch ? p->side : stream;
+        hipStream_t zs = first_launch ? side : stream;
         const int b1 = std::min(b0 + batch, mhi + 1);
         const int nf = b1 - b0;
         const int chunk = (nf + p->tune.chunks - 1) / std::max(1, p->tune.chunks);
@@ -978,12 +989,13 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
                                            (size_t)p->region_ints, 0, 1, zs));
         }
         touched_hi = std::max(touched_hi, tnew_hi);
-        if (first_launch) BM3D_HIP(hipEventRecord(p->join, p->side));
+        const bool joined = first_launch && side != stream;
+        if (joined) BM3D_HIP(hipEventRecord(p->join, side));
         {
             ProfScope ps(0, stream);
             BM3D_HIP(launch_matching(p, p->groups, nullptr, src, ref, b0, nf, stream));
         }
-        if (first_launch) BM3D_HIP(hipStreamWaitEvent(stream, p->join, 0));
+        if (joined) BM3D_HIP(hipStreamWaitEvent(stream, p->join, 0));
         fa.eval_lo = b0;
         fa.eval_hi = b1;
         fa.chunk = chunk;
