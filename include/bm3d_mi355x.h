@@ -49,7 +49,7 @@ extern "C" {
 
 #define BM3D_PRECISION_FP32 0  /* all FP32 on VALU                          */
 #define BM3D_PRECISION_MIXED 1 /* FP16 block-matching pre-scoring, exact
-                                  FP32 rescoring of the best 2k             */
+                                  FP32 rescoring of the best few            */
 
 /* Block distance (the bm_error_s option of the NVRTC variant,
  * rtc_source/kernel.hpp:363-504; parsing rtc_source/source.cpp:854-875). */
@@ -174,7 +174,7 @@ int bm3d_match(bm3d_plan *plan, unsigned int *d_groups, const bm3d_clip *src, co
 long long bm3d_num_blocks(const bm3d_plan *plan);
 
 /* Block-matching kernel of the plan: 1 for the mixed-precision kernel
- * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best 2k),
+ * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best few),
  * 0 for the FP32 kernel (precision FP32, or a search window too large for
  * the mixed kernel's 12-bit scan indices). No counterpart in the reference. */
 int bm3d_plan_matching_kernel(const bm3d_plan *plan);

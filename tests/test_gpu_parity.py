@@ -330,10 +330,10 @@ def test_execute_vagg_equals_stacks_then_vaggregate(case, extractor_exp):
 
 
 # ---------------------------------------------------------- mixed precision
-# The mixed path pre-scores candidates in FP16 and rescores the best 2k
-# exactly (k = 8 spatial, k = ps_num temporal); its lists equal the FP32
-# path's except where an exact top-k candidate ranks below 2k approximately
-# (in practice at exact ties). Slots too wide for FP16 (or not finite) are
+# The mixed path pre-scores candidates in FP16 and rescores the best K
+# exactly (K = 10 spatial; 4, 8 or 16 per temporal frame for ps_num up to 2,
+# 4 or 8); its lists equal the FP32 path's except where an exact top-k
+# candidate ranks below K approximately (in practice at near ties). Slots too wide for FP16 (or not finite) are
 # searched exactly, so those inputs must match bit for bit.
 
 def _mixed_input(kind, noisy):

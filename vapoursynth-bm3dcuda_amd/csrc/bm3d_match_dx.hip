@@ -2,10 +2,11 @@
 //
 // The search of source/kernel.cu:432-647 (cpu_source/source.cpp:143-321)
 // with the candidate distances pre-scored in FP16 and only the best few
-// rescored exactly in the reference's FP32 order (12 for the 8 of the spatial
-// search, 2 ps_num for the ps_num kept of each temporal frame: only the first ps_num entries
-// of a frame's list are merged into the group and centre the next frame's
-// windows, source/kernel.cu:568-611).
+// rescored exactly in the reference's FP32 order (10 for the 8 of the
+// spatial search; 4, 8 or 16, at least 2 ps_num, for the ps_num kept of each
+// temporal frame: only the first ps_num entries of a frame's list are merged
+// into the group and centre the next frame's windows,
+// source/kernel.cu:568-611).
 //
 // Layout as in the FP32 kernel: one lane per reference block, 16 x TBY
 // blocks per work-group, the work-group's search windows staged in LDS one
@@ -14,17 +15,18 @@
 // offset mu (two copies, pairs starting on even and on odd rows), so a
 // candidate column of 8 rows is 4 packed registers. Per candidate:
 //
-//   D  = <r', c'>            32 v_dot2c_f32_f16 (FP32 accumulation)
 //   N  = |c'|^2              sliding sum of per-column v_dot2c sums
-//   A  = (|r'|^2 + N) / 2 - D  half the squared distance up to FP16 rounding
+//   A  = (|r'|^2 + N) / 2 - <r', c'>   one chain of 32 v_dot2c_f32_f16 from
+//                            the first term (FP32 accumulation): half the
+//                            squared distance up to FP16 rounding
 //
 // A is packed with the candidate's scan index into a 32-bit key (A's
-// exponent and 11 mantissa bits above a 12-bit index) and kept by a
-// branch-free min/max cascade of the 2k smallest keys. The kept candidates
-// are rescored exactly and ordered by (distance, scan index), which is the
-// order the reference's strict sorted insertion in scan order produces; the
-// list equals the FP32 path's unless an exact top-k candidate ranks below 2k
-// approximately. A work-group tile whose pixels are not finite or reach
+// exponent and 11 mantissa bits above a 12-bit index); the K smallest keys
+// are kept by branch-free min/max merges of 8 keys at a time. The kept
+// candidates are rescored exactly and ordered by (distance, scan index),
+// which is the order the reference's strict sorted insertion in scan order
+// produces; the list equals the FP32 path's unless an exact top-k candidate
+// ranks below K approximately. A work-group tile whose pixels are not finite or reach
 // beyond +-8 around mu (the centre pixel of the slot's frame) is searched
 // exhaustively in FP32, exactly as the FP32 kernel.
 
