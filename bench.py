@@ -466,7 +466,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the two-pass V-BM3D path shards by frame range, and "
-              "this build runs and reports the single-GPU share only (use --gpus 1).",
+              "this library runs and reports the single-GPU share only (use --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
 

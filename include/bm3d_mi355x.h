@@ -45,7 +45,7 @@ extern "C" {
 #define BM3D_OK 0
 #define BM3D_EINVAL 1       /* invalid argument (message says which)       */
 #define BM3D_EHIP 2         /* HIP runtime error                            */
-#define BM3D_EUNSUPPORTED 3 /* valid for the reference, outside this build  */
+#define BM3D_EUNSUPPORTED 3 /* valid for the reference, not supported here  */
 
 #define BM3D_PRECISION_FP32 0  /* all FP32 on VALU                          */
 #define BM3D_PRECISION_MIXED 1 /* FP16 block-matching pre-scoring, exact

@@ -4,7 +4,8 @@
  * A plain-C restatement of the reference algorithm
  * (WolframRhodium/VapourSynth-BM3DCUDA: cpu_source/source.cpp:91-911 and
  * source/kernel.cu:55-757), used by tests/, by smoke() and by the CPU
- * baseline leg of bench.py as the checker. The product never links it.
+ * baseline leg of bench.py, to check results against. The product never
+ * links it.
  *
  * Built with -ffp-contract=off so that every fused multiply-add is an
  * explicit FMA()/fma() call and every other product and sum rounds on its

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-bench.py use this module, always as the checker. It wraps
+bench.py use this module, and only to check results against. It wraps
 ``oracle/_build/libbm3d_oracle.so`` (built from ``bm3d_oracle.c`` by
 ``oracle/build_oracle.py``) and restates the host-side pieces of the
 reference plugin that sit around the kernel:
