@@ -384,8 +384,13 @@ def test_mixed_groups_agree_fp32(case, kind):
     dict(block_step=2, bm_range=12, radius=2, ps_num=3, ps_range=5),
     dict(block_step=7, bm_range=6, radius=1, ps_num=8, ps_range=1),
     dict(block_step=1, bm_range=2, radius=3, ps_num=1, ps_range=2),
-], ids=["bench", "bs2-R12-r2", "bs7-ps8", "bs1-r3"])
+    dict(block_step=3, bm_range=9, radius=1, ps_num=2, ps_range=4),
+    dict(block_step=6, bm_range=9, radius=1, ps_num=4, ps_range=3),
+], ids=["bench", "bs2-R12-r2", "bs7-ps8", "bs1-r3", "bs3", "bs6-ps4"])
 def test_mixed_groups_agree_fp32_larger(prm):
+    """Interior windows take the matcher's unchecked scan, border windows the
+    checked one; block steps 3, 6 and 7 run its second kernel variant (block
+    step not dividing 8)."""
     core = _gpu()
     H, W, N = 200, 264, 5
     _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=11, motion=True)
