@@ -93,6 +93,13 @@ typedef struct bm3d_desc {
     int bm_error;        /* BM3D_BM_* */
     int transform_2d;    /* BM3D_TRANSFORM_*: along block rows and columns */
     int transform_1d;    /* BM3D_TRANSFORM_*: along the group's members    */
+    /* Precision MIXED: candidates kept per temporal frame for exact
+     * rescoring. 0 = the smallest of 4, 8, 16 that is at least 2 ps_num;
+     * else 4, 8 or 16, at least 2 ps_num. More keys: group lists closer to
+     * the FP32 path's, slower matching (8 at ps_num 2: the FP32 lists on a
+     * 1080p test clip, 14% more matching time). No counterpart in the
+     * reference. */
+    int mixed_temporal_keys;
 } bm3d_desc;
 
 /* A clip in device memory: frame f, plane p starts at

@@ -413,6 +413,32 @@ def test_mixed_groups_agree_fp32_larger(prm):
         assert same >= 0.995
 
 
+def test_mixed_temporal_keys_option():
+    """desc.mixed_temporal_keys: keeping more candidates per temporal frame
+    for exact rescoring keeps a superset, so every list that matched the FP32
+    path's still does; 0 means 4 at ps_num 2. Values other than 0, 4, 8, 16,
+    or below 2 ps_num, are rejected."""
+    core = _gpu()
+    H, W, N = 200, 264, 5
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=12, motion=True)
+    src = torch.from_numpy(noisy).cuda()
+    prm = dict(block_step=4, bm_range=9, radius=1, ps_num=2, ps_range=4)
+    g32 = core.Plan(W, H, 15.0, **prm).match(src).cpu().numpy()
+    lists, agree = {}, {}
+    for kt in (0, 4, 8, 16):
+        pmx = core.Plan(W, H, 15.0, precision="mixed", mixed_temporal_keys=kt, **prm)
+        assert pmx.uses_mixed_kernel()
+        lists[kt] = pmx.match(src).cpu().numpy()
+        agree[kt] = np.all(lists[kt] == g32, axis=-1)
+    print({k: float(v.mean()) for k, v in agree.items()})
+    np.testing.assert_array_equal(lists[0], lists[4])
+    assert not np.any(agree[4] & ~agree[8]) and not np.any(agree[8] & ~agree[16])
+    assert agree[8].mean() >= 0.9999
+    for bad, pn in [(6, 2), (32, 2), (4, 3), (-4, 1)]:
+        with pytest.raises(Exception, match="mixed_temporal_keys"):
+            core.Plan(W, H, 15.0, precision="mixed", mixed_temporal_keys=bad, **dict(prm, ps_num=pn))
+
+
 def test_mixed_two_pass_within_005db_of_fp64():
     """The mixed path's quality bar: two-pass V-BM3D through it is within
     0.05 dB PSNR of the FP64 oracle running the same pipeline."""

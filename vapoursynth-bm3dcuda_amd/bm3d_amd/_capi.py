@@ -34,6 +34,7 @@ class Desc(ctypes.Structure):
         ("chroma", c_int), ("final_", c_int), ("extractor_exp", c_int),
         ("precision", c_int),
         ("bm_error", c_int), ("transform_2d", c_int), ("transform_1d", c_int),
+        ("mixed_temporal_keys", c_int),
     ]
 
 

@@ -50,7 +50,7 @@ def _option(table, value, name):
 
 def _desc(width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2, ps_range=4, chroma=False,
           final=False, extractor_exp=0, precision="fp32", stride=None, bm_error="ssd", transform_2d="dct",
-          transform_1d="dct"):
+          transform_1d="dct", mixed_temporal_keys=0):
     sig = list(sigma) if isinstance(sigma, (list, tuple)) else [sigma]
     sig = (sig + [sig[-1]] * 3)[:3]
     d = C.Desc()
@@ -65,6 +65,7 @@ def _desc(width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2, ps
     d.bm_error = _option(BM_ERRORS, bm_error, "bm_error_s")
     d.transform_2d = _option(TRANSFORMS, transform_2d, "transform_2d_s")
     d.transform_1d = _option(TRANSFORMS, transform_1d, "transform_1d_s")
+    d.mixed_temporal_keys = int(mixed_temporal_keys)
     return d
 
 
@@ -74,9 +75,9 @@ class Plan:
 
     def __init__(self, width, height, sigma, block_step=8, bm_range=9, radius=0, ps_num=2,
                  ps_range=4, chroma=False, final=False, extractor_exp=0, precision="fp32",
-                 stride=None, bm_error="ssd", transform_2d="dct", transform_1d="dct"):
+                 stride=None, bm_error="ssd", transform_2d="dct", transform_1d="dct", mixed_temporal_keys=0):
         d = _desc(width, height, sigma, block_step, bm_range, radius, ps_num, ps_range, chroma, final,
-                  extractor_exp, precision, stride, bm_error, transform_2d, transform_1d)
+                  extractor_exp, precision, stride, bm_error, transform_2d, transform_1d, mixed_temporal_keys)
         self.desc = d
         self.nplanes = 3 if chroma else 1
         self.radius = int(radius)
@@ -221,7 +222,7 @@ class Batch:
     """Many clips denoised together (bm3d_batch_*): each job is a dict with
     the Plan parameters (width, height, sigma, block_step, bm_range, radius,
     ps_num, ps_range, chroma, extractor_exp, precision, bm_error,
-    transform_2d, transform_1d), `src` (a clip
+    transform_2d, transform_1d, mixed_temporal_keys), `src` (a clip
     tensor), `passes` (1: BM3Dv2; 2: two-pass BM3Dv2) and optional `out` /
     `basic` tensors shaped like src (allocated when absent). execute()
     replays the batch's captured graph."""
@@ -242,7 +243,7 @@ class Batch:
                 basic = torch.empty_like(src)
             plan_kw = {k: j[k] for k in ("sigma", "block_step", "bm_range", "radius", "ps_num", "ps_range",
                                          "chroma", "extractor_exp", "precision", "bm_error", "transform_2d",
-                                         "transform_1d") if k in j}
+                                         "transform_1d", "mixed_temporal_keys") if k in j}
             d = _desc(w, h, **plan_kw)
             arr[i].desc = d
             arr[i].passes = passes

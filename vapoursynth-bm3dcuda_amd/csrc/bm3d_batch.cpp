@@ -58,7 +58,7 @@ struct Launch {
     std::vector<int> jobs;
     int pass = 0;
     int block = 0;  // threads (matching) or waves (filtering)
-    int ps_num = 0, final_ = 0, chroma = 0;
+    int keys = 0, final_ = 0, chroma = 0;  // keys: the mixed matcher's temporal keys
     int variant = 0;  // matching: block distance (FP32) or block step divides 8 (mixed);
                       // filtering: the FP32 generic-transform variant
     size_t lds = 0;
@@ -213,7 +213,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
             std::map<std::tuple<int, int, int, int>, std::vector<int>> mgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
-                const int keys = p->use_dx ? bm3d::dx_temporal_keys(p->d.ps_num) : 0;
+                const int keys = p->use_dx ? bm3d::plan_temporal_keys(p) : 0;
                 const int variant = p->use_dx ? (bm3d::dx_block_step_divides_8(p->d.block_step) ? 1 : 0)
                                               : p->d.bm_error;
                 mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, variant}].push_back(i);
@@ -237,7 +237,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy * j.job.src.num_frames;
                     l.lds = std::max(l.lds, p->use_dx ? p->dx_lds_bytes : p->lds_bytes);
-                    l.ps_num = p->d.ps_num;
+                    l.keys = bm3d::plan_temporal_keys(p);
                     args.push_back(a);
                     l.jobs.push_back(i);
                 }
@@ -344,7 +344,7 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                             l.wgs, l.block, l.lds, l.variant, s);
         case 2:
             return bm3d::launch_match_dx_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
-                                               l.wgs, l.block, l.lds, l.ps_num, l.variant != 0, s);
+                                               l.wgs, l.block, l.lds, l.keys, l.variant != 0, s);
         case 3:
             return bm3d::launch_filter_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids, n,
                                              l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);

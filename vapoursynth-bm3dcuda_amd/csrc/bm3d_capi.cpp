@@ -170,9 +170,9 @@ TuneCache& tune_cache() {
 std::string tune_key(const bm3d_plan* p) {
     const bm3d_desc& d = p->d;
     char buf[192];
-    std::snprintf(buf, sizeof buf, "w%d h%d s%d bs%d R%d r%d pn%d pr%d c%d f%d p%d e%d t%d%d", d.width, d.height,
-                  d.stride, d.block_step, d.bm_range, d.radius, d.ps_num, d.ps_range, d.chroma, d.final_, d.precision,
-                  d.bm_error, d.transform_2d, d.transform_1d);
+    std::snprintf(buf, sizeof buf, "w%d h%d s%d bs%d R%d r%d pn%d pr%d c%d f%d p%d e%d t%d%d k%d", d.width,
+                  d.height, d.stride, d.block_step, d.bm_range, d.radius, d.ps_num, d.ps_range, d.chroma, d.final_,
+                  d.precision, d.bm_error, d.transform_2d, d.transform_1d, bm3d::plan_temporal_keys(p));
     return buf;
 }
 
@@ -365,6 +365,7 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.ncand = ncand;
     m.tile_hp2 = p->tile_hp2;
     m.cost = d.bm_error;
+    m.kt = bm3d::plan_temporal_keys(p);
     return m;
 }
 
@@ -486,6 +487,9 @@ namespace bm3d {
 
 int plan_apply_tuning(bm3d_plan* p, const bm3d_tuning& t) { return apply_tuning(p, t); }
 const char* plan_tuning_error(const bm3d_tuning& t) { return tuning_range_error(t); }
+int plan_temporal_keys(const bm3d_plan* p) {
+    return p->d.mixed_temporal_keys ? p->d.mixed_temporal_keys : dx_temporal_keys(p->d.ps_num);
+}
 MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
     return match_args(p, nullptr, nullptr, src, ref, 0);
 }
@@ -530,6 +534,10 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     if (d.transform_2d < 0 || d.transform_2d >= bm3d::kNumTransforms || d.transform_1d < 0 ||
         d.transform_1d >= bm3d::kNumTransforms)
         return fail(BM3D_EINVAL, "BM3D: unknown transform");
+    if (d.mixed_temporal_keys != 0 &&
+        ((d.mixed_temporal_keys != 4 && d.mixed_temporal_keys != 8 && d.mixed_temporal_keys != 16) ||
+         d.mixed_temporal_keys < 2 * d.ps_num))
+        return fail(BM3D_EINVAL, "BM3D: mixed_temporal_keys must be 0, or 4, 8 or 16 and at least 2 ps_num");
 
     auto* p = new bm3d_plan;
     p->d = d;

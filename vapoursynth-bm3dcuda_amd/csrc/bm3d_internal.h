@@ -72,6 +72,9 @@ namespace bm3d {
 int plan_apply_tuning(bm3d_plan* p, const bm3d_tuning& t);
 const char* plan_tuning_error(const bm3d_tuning& t);
 MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
+// Keys the mixed matcher keeps per temporal frame: desc.mixed_temporal_keys,
+// or dx_temporal_keys(ps_num) when that is 0.
+int plan_temporal_keys(const bm3d_plan* p);
 FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
 VTargetArgs plan_vtarget_args(const bm3d_plan* p);
 

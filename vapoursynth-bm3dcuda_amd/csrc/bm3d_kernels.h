@@ -48,6 +48,7 @@ struct MatchArgs {
     uint32_t* groups;        // [frames][nblocks][8] (may be null when counting)
     unsigned long long* ncand;  // non-null: count in-plane candidates
     int cost;                // block distance, 0 .. kNumCosts - 1 (the mixed kernel: 0 only)
+    int kt;                  // mixed kernel: keys kept per temporal frame (4, 8 or 16)
 };
 
 struct FilterArgs {
@@ -178,10 +179,10 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 // variant; lds_bytes covers the largest job.
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, int cost, hipStream_t s);
-// (every job of one launch has the same dx_temporal_keys, and div8 is
+// (every job of one launch has the same MatchArgs::kt, and div8 is
 // dx_block_step_divides_8 of each job's block step)
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int ps_num, bool div8, hipStream_t s);
+                                 size_t lds_bytes, int kt, bool div8, hipStream_t s);
 // generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);
@@ -202,7 +203,8 @@ hipError_t set_match_lds_limit(size_t bytes);
 // launch_match; LDS bytes as returned by dx_lds_bytes.
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_dx_lds_limit(size_t bytes);
-// Keys kept per temporal frame (at least 2 ps_num).
+// Default keys kept per temporal frame: the smallest of 4, 8, 16 that is at
+// least 2 ps_num.
 int dx_temporal_keys(int ps_num);
 // The mixed matcher has one kernel variant for block steps dividing 8 and
 // one for the others.

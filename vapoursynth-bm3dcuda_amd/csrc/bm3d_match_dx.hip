@@ -657,7 +657,7 @@ hipError_t set_dx_limit(size_t bytes) {
 }
 template <bool kDiv8>
 hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
-    switch (dx_temporal_keys(a.ps_num)) {
+    switch (a.kt) {
         case 4: return launch_dx<4, kDiv8>(a, tby, lds_bytes, frames, s);
         case 8: return launch_dx<8, kDiv8>(a, tby, lds_bytes, frames, s);
         default: return launch_dx<16, kDiv8>(a, tby, lds_bytes, frames, s);
@@ -665,8 +665,8 @@ hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int fra
 }
 template <bool kDiv8>
 hipError_t launch_dx_multi_keys(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                size_t lds_bytes, int ps_num, hipStream_t s) {
-    switch (dx_temporal_keys(ps_num)) {
+                                size_t lds_bytes, int kt, hipStream_t s) {
+    switch (kt) {
         case 4: return launch_dx_multi<4, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
         case 8: return launch_dx_multi<8, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
         default: return launch_dx_multi<16, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
@@ -684,9 +684,9 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
 }
 
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int ps_num, bool div8, hipStream_t s) {
-    return div8 ? launch_dx_multi_keys<true>(jobs, grids, njobs, wgs, threads, lds_bytes, ps_num, s)
-                : launch_dx_multi_keys<false>(jobs, grids, njobs, wgs, threads, lds_bytes, ps_num, s);
+                                 size_t lds_bytes, int kt, bool div8, hipStream_t s) {
+    return div8 ? launch_dx_multi_keys<true>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s)
+                : launch_dx_multi_keys<false>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
 }
 
 hipError_t set_match_dx_lds_limit(size_t bytes) {
