@@ -271,6 +271,8 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
     // (a compile-time choice: a branch here would split the loop body and
     // cost register copies of every column loaded)
     const int inc8 = kDiv8 ? (8 / g.bs) * g.hp2 : 0;
+    // ~kIdxMask through an opaque register move: the compiler keeps it in a
+    // vector register instead of copying each scan index into one
     uint32_t hi_mask;
     asm volatile("v_mov_b32 %0, %1" : "=v"(hi_mask) : "i"(~kIdxMask));
     f16x2 ring[8][4];
