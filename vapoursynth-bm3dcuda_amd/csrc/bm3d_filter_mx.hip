@@ -106,12 +106,19 @@ __device__ __forceinline__ float wave_max_abs(float m) {
 // 8 members (|W| < 23200). A constant
 // offset only moves the DC coefficient, which both shrinkage rules pass
 // unchanged, so mu is added back exactly after the inverse.
-__device__ __forceinline__ void forward_mx(f4 (&z)[4], float& mu, float& s, const MxOps& o,
-                                           const float* __restrict__ mp, int stride, int q, int c) {
+// The lane's two rows of its block (rows q and q + 4, 16 bytes each half).
+__device__ __forceinline__ void load_rows(f4u (&v)[4], const float* __restrict__ mp, int stride, int q) {
     const float* r0 = mp + (size_t)q * stride;
     const float* r1 = r0 + (size_t)4 * stride;
-    f4u v[4] = {*reinterpret_cast<const f4u*>(r0), *reinterpret_cast<const f4u*>(r0 + 4),
-                *reinterpret_cast<const f4u*>(r1), *reinterpret_cast<const f4u*>(r1 + 4)};
+    v[0] = *reinterpret_cast<const f4u*>(r0);
+    v[1] = *reinterpret_cast<const f4u*>(r0 + 4);
+    v[2] = *reinterpret_cast<const f4u*>(r1);
+    v[3] = *reinterpret_cast<const f4u*>(r1 + 4);
+}
+
+// forward_mx on rows already loaded by load_rows (v is overwritten).
+__device__ __forceinline__ void forward_mx_rows(f4 (&z)[4], float& mu, float& s, const MxOps& o, f4u (&v)[4],
+                                                int c) {
     const float mu0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[0][0]), 0));
     const float mu1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[0][0]), 8));
     mu = (c >> 3) ? mu1 : mu0;
@@ -143,6 +150,13 @@ __device__ __forceinline__ void forward_mx(f4 (&z)[4], float& mu, float& s, cons
         y = mfma32(a1, o.kf[t][1], y);
         z[t] = mfma16(o.dA, cvt4(y), zero);
     }
+}
+
+__device__ __forceinline__ void forward_mx(f4 (&z)[4], float& mu, float& s, const MxOps& o,
+                                           const float* __restrict__ mp, int stride, int q, int c) {
+    f4u v[4];
+    load_rows(v, mp, stride, q);
+    forward_mx_rows(z, mu, s, o, v, c);
 }
 
 // Inverse 3-D transform of shrunk coefficients z (layout of forward_mx):
@@ -230,9 +244,13 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
                 // Wiener shrinkage (source/kernel.cu:284-326): coefficients
                 // from the basic estimate's group, applied to the noisy group.
                 // The DC coefficient is 1, so the offsets pass unchanged.
+                // both groups' rows are requested before either transform
+                f4u vb[4], vn[4];
+                load_rows(vb, mp + (a.ref - a.src), a.stride, q);
+                load_rows(vn, mp, a.stride, q);
                 f4 cf[4];
                 float mub, sb;
-                forward_mx(cf, mub, sb, o, mp + (a.ref - a.src), a.stride, q, c);
+                forward_mx_rows(cf, mub, sb, o, vb, c);
                 const float s2 = (sigma * sigma) / (sb * sb);  // sigma^2 in units of the scaled residual
                 float sum = 0.0f;
 #pragma unroll
@@ -253,7 +271,7 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
                 const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sum), 48));
                 w0 = 1.0f / (r0 + r1);
                 w1 = 1.0f / (r2 + r3);
-                forward_mx(zc, mu, sc, o, mp, a.stride, q, c);
+                forward_mx_rows(zc, mu, sc, o, vn, c);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
 #pragma unroll
