@@ -73,6 +73,41 @@ def test_ssd_matches_float64():
     assert O.ssd(ref, img, 5, 3) == 0.0
 
 
+def _ssd32(r, c):
+    return O.ssd(r.astype(np.float32), c.astype(np.float32), 0, 0)
+
+
+def test_ssd_summation_structure():
+    """The FP32 distance follows cpu_source/source.cpp:98-103,123-136: per
+    column an FMA chain over the even rows and one over the odd rows, their
+    sum, then ((s0+s1)+(s2+s3))+((s4+s5)+(s6+s7)). That order is bitwise
+    invariant under exactly the swaps that permute its own additions:
+    exchanging the even and odd rows (the two chains), exchanging columns
+    2c and 2c+1, the column pairs within a half, and the two halves. Orders a
+    slip would plausibly produce (one chain over all rows, row-major sums, a
+    sequential column sum) fail these on random data; swaps outside that
+    group (rows 0 and 2) must change some result, so the chain really is
+    sequential."""
+    rng = np.random.default_rng(41)
+    rows_eo = [1, 0, 3, 2, 5, 4, 7, 6]
+    cols = {"pairs": [1, 0, 3, 2, 5, 4, 7, 6], "quads": [2, 3, 0, 1, 6, 7, 4, 5],
+            "halves": [4, 5, 6, 7, 0, 1, 2, 3]}
+    changed_rows02 = changed_cols12 = 0
+    for _ in range(400):
+        scale = 2.0 ** rng.integers(-6, 3)
+        r = rng.uniform(-1, 1, (8, 8)) * scale
+        c = rng.uniform(-1, 1, (8, 8)) * scale
+        base = _ssd32(r, c)
+        assert _ssd32(r[rows_eo], c[rows_eo]) == base
+        for perm in cols.values():
+            assert _ssd32(r[:, perm], c[:, perm]) == base
+        p02 = [2, 1, 0, 3, 4, 5, 6, 7]
+        changed_rows02 += _ssd32(r[p02], c[p02]) != base
+        p12 = [0, 2, 1, 3, 4, 5, 6, 7]
+        changed_cols12 += _ssd32(r[:, p12], c[:, p12]) != base
+    assert changed_rows02 > 20 and changed_cols12 > 20
+
+
 # ----------------------------------------------- selection (order-free)
 def _brute_lists(slots, bs, R, radius, ps_num, ps_range):
     """Order-free restatement: every list is the 8 smallest keys
@@ -310,3 +345,123 @@ def test_vaggregate_border_example():
     win = [stacks[0], stacks[0], stacks[1]]
     out = O.vaggregate(win, np.zeros((1, 1, 1), np.float32), 0, 5, 1, [True])
     assert out[0, 0, 0] == np.float32((1 + 1 + 10) / 3)
+
+
+# ---------------------------------------------------- extractor pre-rounding
+# The GPU kernel pre-rounds every weighted estimate and weight as
+# v <- (v + E) - E with E = 2^extractor_exp (source/kernel.cu:740-742). The
+# oracle's (v + E) - E is pinned against binary32 round-to-nearest-even
+# evaluated in exact rational arithmetic, against a closed form on
+# [-E/2, E), and through the grid every pre-rounded value lies on.
+from fractions import Fraction
+
+
+def _rne32(x: Fraction) -> Fraction:
+    """x rounded to the nearest binary32 value, ties to even (exact)."""
+    if x == 0:
+        return Fraction(0)
+    a = abs(x)
+    k = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** k > a:
+        k -= 1
+    grid = Fraction(2) ** (max(k, -126) - 23)
+    q = a / grid
+    n = q.numerator // q.denominator
+    rem = q - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return (n * grid) if x > 0 else -(n * grid)
+
+
+def _preround_samples(e, rng):
+    E = 2.0 ** e
+    g = 2.0 ** (e - 24)
+    v = [rng.uniform(-E, E, 300), rng.uniform(-E / 2, E / 2, 200), rng.uniform(-8 * E, 8 * E, 200),
+         rng.normal(size=100) * 2.0 ** rng.integers(-40, 0, 100),      # tiny values
+         (rng.integers(-2 ** 23, 2 ** 23, 100) + 0.5) * g,               # ties on the 2^(e-24) grid
+         (rng.integers(-2 ** 22, 2 ** 22, 100) + 0.5) * 2 * g,           # ties on the 2^(e-23) grid
+         -E / 2 + rng.integers(-50, 50, 100) * g / 4,                     # around -E/2
+         np.array([0.0, -0.0, E, -E, E / 2, -E / 2, 2 * E, -2 * E])]
+    return np.concatenate(v).astype(np.float32)
+
+
+@pytest.mark.parametrize("e", [0, 3, 6, 10, 20])
+def test_preround_is_binary32_rne(e):
+    """(v + E) - E == rne32(rne32(v + E) - E), computed exactly."""
+    E = np.float32(2.0 ** e)
+    rng = np.random.default_rng(100 + e)
+    for v in _preround_samples(e, rng):
+        want = _rne32(_rne32(Fraction(float(v)) + Fraction(float(E))) - Fraction(float(E)))
+        got = O.preround(v, E)
+        assert Fraction(got) == want, (float(v), e, got, float(want))
+
+
+@pytest.mark.parametrize("e", [0, 4, 9])
+def test_preround_closed_form(e):
+    """On [-E/2, E) the pre-rounding is v rounded half-to-even to the grid
+    2^(e-24) (v < 0) or 2^(e-23) (v >= 0): the ulp of v + E, exact in the
+    subtraction (Sterbenz)."""
+    E = 2.0 ** e
+    rng = np.random.default_rng(7 + e)
+    v = np.concatenate([rng.uniform(-E / 2, E, 2000),
+                        (rng.integers(-2 ** 22, 2 ** 23, 300) + 0.5) * 2.0 ** (e - 24)]).astype(np.float32)
+    v = v[(v >= -E / 2) & (v < E)].astype(np.float64)
+    g = np.where(v >= 0, 2.0 ** (e - 23), 2.0 ** (e - 24))
+    want = np.rint(v / g) * g   # exact: power-of-two scaling, rint ties to even
+    got = np.array([O.preround(x, E) for x in v])
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("e", [0, 5, 12])
+def test_preround_values_on_unit_grid(e):
+    """Every pre-rounded value is a whole multiple of 2^(e-24), the unit of
+    the kernels' fixed-point accumulation."""
+    rng = np.random.default_rng(31 + e)
+    unit = 2.0 ** (e - 24)
+    for v in _preround_samples(e, rng):
+        q = O.preround(v, 2.0 ** e) / unit
+        assert float(q).is_integer(), (float(v), e)
+
+
+def _coverage(groups, bs, H, W, tw):
+    """Members covering each pixel of each slot: counts [tw, H, W]."""
+    cnt = np.zeros((tw, H, W), np.int64)
+    for x, y, z in groups.reshape(-1, 3):
+        cnt[z, y:y + 8, x:x + 8] += 1
+    return cnt
+
+
+@pytest.mark.parametrize("final", [False, True])
+@pytest.mark.parametrize("e", [2, 8])
+def test_extractor_stacks_on_grid_and_bounded(final, e):
+    """With extractor_exp = e every num and den entry of an oracle stack is a
+    multiple of 2^(e-24) (a dropped rounding of w or of w * est breaks it),
+    and |stack_E - stack_0| <= contributions * 2^(e-24) plus float
+    accumulation slack (each contribution moves by at most half its grid)."""
+    H, W, N, r = 32, 40, 3, 1
+    bs, R, pn, pr = 4, 4, 2, 2
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=5, motion=True)
+    clip = noisy[:, 0]
+    win = np.stack([clip[min(max(1 - r + z, 0), N - 1)] for z in range(2 * r + 1)])[None]
+    refw = None
+    if final:
+        base = O.bm3d_window(win, None, [O.scale_sigma(15.0, False)], bs, R, r, pn, pr, False, False)
+        refw = np.where(base[:, :, 1] > 0, base[:, :, 0] / np.maximum(base[:, :, 1], 1e-30), win)
+    sig = [O.scale_sigma(15.0, final)]
+    E = float(2.0 ** e)
+    sE, groups, _ = O.bm3d_window(win, refw, sig, bs, R, r, pn, pr, False, final, E, want_groups=True)
+    s0 = O.bm3d_window(win, refw, sig, bs, R, r, pn, pr, False, final, 0.0)
+    unit = 2.0 ** (e - 24)
+    q = sE.astype(np.float64) / unit
+    assert np.array_equal(q, np.rint(q)), "stack entry off the 2^(e-24) grid"
+    # control: without pre-rounding the weights are not all on that grid
+    q0 = s0[:, :, 1].astype(np.float64) / unit
+    assert not np.array_equal(q0, np.rint(q0))
+    cnt = _coverage(groups, bs, H, W, 2 * r + 1)[None]
+    for part in (0, 1):
+        a = sE[:, :, part].astype(np.float64)
+        b = s0[:, :, part].astype(np.float64)
+        slack = cnt * 2.0 ** -23 * np.maximum(np.abs(a), np.abs(b))
+        assert np.all(np.abs(a - b) <= cnt * unit + slack)
+    if e >= 8:
+        assert np.abs(sE - s0).max() > 0  # the rounding is visible at this E
