@@ -49,7 +49,7 @@ struct Job {
     int nplanes = 1;
     int wave = 0;
     // scratch of the job within its wave's arena (bytes)
-    size_t corner_off = 0, range_off = 0, groups_off = 0, scratch_off = 0;
+    size_t corner_off = 0, wrap_off = 0, range_off = 0, groups_off = 0, scratch_off = 0;
 };
 
 // One kernel launch of the graph.
@@ -59,6 +59,7 @@ struct Launch {
     int pass = 0;
     int block = 0;  // threads (matching) or waves (filtering)
     int keys = 0, final_ = 0, chroma = 0;  // keys: the mixed matcher's temporal keys
+    int any_extractor = 0;                 // mixed filtering: some job has extractor_exp
     int variant = 0;  // matching: block distance (FP32) or block step divides 8 (mixed);
                       // filtering: the FP32 generic-transform variant
     size_t lds = 0;
@@ -76,7 +77,7 @@ struct bm3d_batch {
     long long wave_budget = kDefaultWaveBudget;
     size_t arena_bytes = 0;
     char* arena = nullptr;
-    std::vector<std::pair<size_t, size_t>> wave_zero;  // per wave: bytes of corner maps + ranges from 0
+    std::vector<std::pair<size_t, size_t>> wave_zero;  // per wave: bytes of corner and wrap maps + ranges from 0
     char* tables = nullptr;
     size_t tables_bytes = 0;
     std::vector<std::vector<Launch>> wave_launches;  // per wave, in order
@@ -112,12 +113,16 @@ void layout(bm3d_batch* b) {
     std::vector<int> wave_jobs;
     auto close_wave = [&]() {
         if (wave_jobs.empty()) return;
-        // corner maps and ranges first (zeroed together), then groups and regions
+        // corner maps, wrap maps and ranges (with the wrap flag after each
+        // range) first, zeroed together before each pass; then groups and
+        // regions
         size_t off = 0;
         for (int i : wave_jobs) {
             Job& j = b->jobs[i];
+            const size_t map = align_up((size_t)j.job.src.num_frames * j.nplanes * j.plan[0]->plane_size * 4, 256);
             j.corner_off = off;
-            off += align_up((size_t)j.job.src.num_frames * j.nplanes * j.plan[0]->plane_size * sizeof(float), 256);
+            j.wrap_off = off + map;
+            off += 2 * map;
         }
         for (int i : wave_jobs) {
             b->jobs[i].range_off = off;
@@ -141,7 +146,7 @@ void layout(bm3d_batch* b) {
         Job& j = b->jobs[i];
         const bm3d_plan* p = j.plan[0];
         const long long N = j.job.src.num_frames;
-        const size_t need = (size_t)N * ((size_t)j.nplanes * p->plane_size * 4 + (size_t)p->nblocks * 32 +
+        const size_t need = (size_t)N * ((size_t)j.nplanes * p->plane_size * 8 + (size_t)p->nblocks * 32 +
                                          (size_t)p->region_ints * 4) + 4096;
         if (!wave_jobs.empty() && used + need > (size_t)b->wave_budget) close_wave();
         j.wave = b->nwaves;
@@ -173,9 +178,9 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
             }
             if (js.empty()) continue;
             auto& L = b->wave_launches[w];
-            auto fx_range = [&](const Job& j) -> float* {
-                return j.job.desc.extractor_exp ? nullptr : reinterpret_cast<float*>(base + j.range_off);
-            };
+            auto range = [&](const Job& j) -> float* { return reinterpret_cast<float*>(base + j.range_off); };
+            auto fx_range = [&](const Job& j) -> float* { return j.job.desc.extractor_exp ? nullptr : range(j); };
+            auto wrap_flag = [&](const Job& j) -> int* { return reinterpret_cast<int*>(base + j.range_off) + 1; };
             // fixed-point range
             {
                 Launch l{};
@@ -185,7 +190,6 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                 std::vector<bm3d::JobGrid> grids;
                 for (int i : js) {
                     const Job& j = b->jobs[i];
-                    if (j.job.desc.extractor_exp) continue;
                     bm3d::AbsmaxArgs a{};
                     a.clip = j.job.src.data;
                     a.frame_stride = j.job.src.frame_stride;
@@ -194,7 +198,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     a.W = j.job.desc.width;
                     a.H = j.job.desc.height;
                     a.stride = j.job.desc.stride;
-                    a.out = fx_range(j);
+                    a.out = range(j);
                     const int gx = std::min(a.H, 64), gy = j.job.src.num_frames * j.nplanes;
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy;
@@ -275,6 +279,9 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     a.corner = reinterpret_cast<float*>(base + j.corner_off);
                     a.scratch = reinterpret_cast<int*>(base + j.scratch_off);
                     a.fx_range = fx_range(j);
+                    a.range = range(j);
+                    a.wrap = reinterpret_cast<int*>(base + j.wrap_off);
+                    a.wrap_flag = wrap_flag(j);
                     a.res = nullptr;
                     a.mode = 1;
                     a.eval_lo = 0;
@@ -286,6 +293,7 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     grids.push_back({l.wgs, p->fntx, p->fnty});
                     l.wgs += p->fntx * p->fnty;
                     l.lds = std::max(l.lds, p->filter_lds_bytes);
+                    l.any_extractor |= p->d.extractor_exp != 0;
                     args.push_back(a);
                     l.jobs.push_back(i);
                 }
@@ -309,6 +317,8 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     a.corner = reinterpret_cast<const float*>(base + j.corner_off);
                     a.ring = N;
                     a.fx_range = fx_range(j);
+                    a.wrap = reinterpret_cast<const int*>(base + j.wrap_off);
+                    a.wrap_flag = wrap_flag(j);
                     a.plane_mask = j.process;
                     a.dst = (passes(j) == 2 && pass == 0) ? j.job.basic : j.job.out;
                     a.dst_frame_stride = j.job.src.frame_stride;
@@ -350,7 +360,8 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                              l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);
         case 4:
             return bm3d::launch_filter_mx_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids,
-                                                n, l.wgs, l.final_ != 0, l.chroma != 0, l.block, l.lds, s);
+                                                n, l.wgs, l.final_ != 0, l.chroma != 0, l.block, l.lds, s,
+                                                l.any_extractor != 0);
         default:
             return bm3d::launch_vagg_targets_multi(reinterpret_cast<const bm3d::VTargetArgs*>(t + l.args_off),
                                                    grids, n, l.wgs, s);

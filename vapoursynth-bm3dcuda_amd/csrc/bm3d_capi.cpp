@@ -292,6 +292,8 @@ bool filter_geometry(bm3d_plan* p, int tbx, int tby, int rm) {
 // with a reduced margin (members beyond it go to global atomics, and
 // bm3d_execute_vagg is unavailable).
 int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
+    if (p->pinned)
+        return fail(BM3D_EINVAL, "BM3D: the plan is held by a captured graph; its launch shapes cannot change");
     if (!match_geometry(p, t.match_rows, t.match_width) && !match_geometry(p, 4, 16))
         return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
     dx_geometry(p);
@@ -316,8 +318,20 @@ int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
 }
 
 // Grows the plan's device buffers to hold at least the given numbers of
-// group-list entries, corner-map floats and region ints.
-int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch) {
+// group-list entries, corner-map floats (and as many wrap counts), region
+// ints and 64-bit partial sums. A call on a capturing stream pins the plan: its buffers must already
+// be large enough (an eager call sizes them), and from then on they never
+// move, since the captured graph holds their addresses.
+int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch, hipStream_t stream,
+                   size_t part = 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    BM3D_HIP(hipStreamIsCapturing(stream, &cs));
+    const bool grow = groups > p->groups_cap || corner > p->corner_cap || scratch > p->scratch_cap ||
+                      part > p->part_cap;
+    if (cs != hipStreamCaptureStatusNone) p->pinned = true;
+    if (grow && p->pinned)
+        return fail(BM3D_EINVAL, "BM3D: the plan's buffers are too small for this call and are held by a captured "
+                                 "graph (size them with an eager call of the largest range before capturing)");
     if (groups > p->groups_cap) {
         if (p->groups) (void)hipFree(p->groups);
         p->groups = nullptr;
@@ -327,9 +341,14 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch) {
     }
     if (corner > p->corner_cap) {
         if (p->corner) (void)hipFree(p->corner);
+        if (p->wrap) (void)hipFree(p->wrap);
         p->corner = nullptr;
+        p->wrap = nullptr;
         p->corner_cap = 0;
         BM3D_HIP(hipMalloc(&p->corner, corner * sizeof(float)));
+        BM3D_HIP(hipMalloc(&p->wrap, corner * sizeof(int)));
+        // counts start at zero: readers use them whenever the flag is set
+        BM3D_HIP(hipMemset(p->wrap, 0, corner * sizeof(int)));
         p->corner_cap = corner;
     }
     if (scratch > p->scratch_cap) {
@@ -338,6 +357,13 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch) {
         p->scratch_cap = 0;
         BM3D_HIP(hipMalloc(&p->scratch, scratch * sizeof(int)));
         p->scratch_cap = scratch;
+    }
+    if (part > p->part_cap) {
+        if (p->part) (void)hipFree(p->part);
+        p->part = nullptr;
+        p->part_cap = 0;
+        BM3D_HIP(hipMalloc(&p->part, part * sizeof(long long)));
+        p->part_cap = part;
     }
     return BM3D_OK;
 }
@@ -381,10 +407,10 @@ hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long l
 const float* fx_range(const bm3d_plan* p) { return p->d.extractor_exp ? nullptr : p->range; }
 
 // Measures the largest finite |sample| of the processed planes of frames
-// lo .. hi of src into p->range (the fixed-point unit of every evaluation
-// and aggregation of the call follows from it).
+// lo .. hi of src into p->range: the fixed-point unit of every evaluation
+// and aggregation of the call follows from it (without extractor_exp), and
+// so does the filters' choice of 64-bit adds (always).
 hipError_t measure_range(bm3d_plan* p, const bm3d_clip* src, int lo, int hi, hipStream_t stream) {
-    if (!fx_range(p)) return hipSuccess;
     hipError_t e = bm3d::launch_zero(p->range, 1, 0, 1, stream);
     if (e != hipSuccess) return e;
     return bm3d::launch_absmax(src->data + (size_t)lo * src->frame_stride, src->frame_stride, src->plane_stride,
@@ -414,6 +440,10 @@ bm3d::FilterArgs filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3
     fa.scratch = p->scratch;
     fa.fx_range = fx_range(p);
     fa.fx_exp = p->fx_exp;
+    fa.wrap = p->wrap;
+    fa.wrap_flag = p->wrap_flag;
+    fa.range = p->range;
+    fa.vbound = p->vbound;
     fa.mx_ops = p->mx_ops;
     fa.t2d = d.transform_2d;
     fa.t1d = d.transform_1d;
@@ -442,6 +472,8 @@ bm3d::VTargetArgs vtarget_args(const bm3d_plan* p, int ring) {
     va.ring = ring;
     va.fx_range = fx_range(p);
     va.fx_exp = p->fx_exp;
+    va.wrap = p->wrap;
+    va.wrap_flag = p->wrap_flag;
     va.W = d.width; va.H = d.height; va.stride = d.stride;
     va.plane_size = p->plane_size;
     va.nplanes = p->nplanes;
@@ -459,6 +491,8 @@ hipError_t evaluate_slots(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* r
                           long long res_frame_stride, long long res_plane_stride, hipStream_t stream) {
     const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
     hipError_t e = bm3d::launch_zero(p->corner, (size_t)nf * corner_eval, 0, 1, stream);
+    if (e == hipSuccess)
+        e = bm3d::launch_zero(reinterpret_cast<float*>(p->wrap), (size_t)nf * corner_eval, 0, 1, stream, p->wrap_flag);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(0, stream);
@@ -574,11 +608,14 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         return rc;
     }
     p->use_dx = p->dx_ok;
-    e = hipMalloc(&p->range, sizeof(float));
+    p->vbound = bm3d::contribution_bound(d.transform_2d, d.transform_1d, p->d.final_ != 0);
+    e = hipMalloc(&p->range, 256);  // the range, then the wrap flag
+    if (e == hipSuccess) e = hipMemset(p->range, 0, 256);
     if (e != hipSuccess) {
         bm3d_plan_destroy(p);
         return hip_fail(e, "hipMalloc");
     }
+    p->wrap_flag = reinterpret_cast<int*>(p->range) + 1;
 
     if (d.precision == BM3D_PRECISION_MIXED) {
         std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);
@@ -600,6 +637,8 @@ void bm3d_plan_destroy(bm3d_plan* p) {
     if (!p) return;
     if (p->groups) (void)hipFree(p->groups);
     if (p->corner) (void)hipFree(p->corner);
+    if (p->wrap) (void)hipFree(p->wrap);
+    if (p->part) (void)hipFree(p->part);
     if (p->scratch) (void)hipFree(p->scratch);
     if (p->mx_ops) (void)hipFree(p->mx_ops);
     if (p->range) (void)hipFree(p->range);
@@ -634,9 +673,9 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
 
     const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
     const long long scratch_eval = (long long)p->tw * p->region_ints;
-    const int batch = batch_limit(p, count, (corner_eval + scratch_eval) * 4 + (long long)p->nblocks * 32);
+    const int batch = batch_limit(p, count, corner_eval * 8 + scratch_eval * 4 + (long long)p->nblocks * 32);
     if (int rc = ensure_buffers(p, (size_t)batch * p->nblocks * 8, (size_t)batch * corner_eval,
-                                (size_t)batch * scratch_eval))
+                                (size_t)batch * scratch_eval, stream))
         return rc;
 
     const bm3d_desc& d = p->d;
@@ -665,6 +704,9 @@ int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long lo
         fz.scratch = p->scratch;
         fz.fx_range = fx_range(p);
         fz.fx_exp = p->fx_exp;
+        fz.wrap = p->wrap;
+        fz.wrap_flag = p->wrap_flag;
+        fz.corner_frame_stride = corner_eval;
         fz.res = res;
         fz.res_frame_stride = res_frame_stride;
         fz.res_plane_stride = res_plane_stride;
@@ -782,6 +824,8 @@ int bm3d_plan_autotune(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref,
     if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
     if (src->num_frames <= 0 || count <= 0 || first_frame < 0 || first_frame + count > src->num_frames)
         return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    if (p->pinned)
+        return fail(BM3D_EINVAL, "BM3D: the plan is held by a captured graph; its launch shapes cannot change");
     hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
     const bm3d_desc& d = p->d;
     const bool fused = d.radius > 0 && p->frm >= p->margin;
@@ -927,16 +971,18 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     // output frames the evaluations add to
     const int tlo_all = std::max(mlo - r, 0), thi_all = std::min(mhi + r, N - 1);
     const long long corner_target = (long long)p->nplanes * p->plane_size;
-    const long long target_bytes = (corner_target + p->region_ints) * 4;
+    // per resident output frame: corner map, wrap counts, 64-bit partial sums, regions
+    const long long target_bytes = corner_target * 16 + p->region_ints * 4;
     const int batch = batch_limit(p, mhi - mlo + 1, target_bytes + (long long)p->nblocks * 32, 2 * r * target_bytes);
     // resident output frames: those a launch adds to (its evaluations +- r)
     const int ring = std::min(batch + 2 * r, thi_all - tlo_all + 1);
     if (int rc = ensure_buffers(p, (size_t)batch * p->nblocks * 8, (size_t)ring * corner_target,
-                                (size_t)ring * p->region_ints))
+                                (size_t)ring * p->region_ints, stream, (size_t)ring * corner_target))
         return rc;
 
     bm3d::FilterArgs fa = filter_args(p, src, ref);
     fa.mode = 1;
+    fa.part = p->part;
     fa.ring = ring;
     fa.target_stride = p->region_ints;
     fa.corner_frame_stride = corner_target;
@@ -945,6 +991,9 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
     va.dst_frame_stride = dst_frame_stride;
     va.dst_plane_stride = dst_plane_stride;
     va.dst_stride = dst_stride;
+    va.part = p->part;
+    // output frames whose contributors span several runs (64-bit sums)
+    std::vector<char> partial(thi_all - tlo_all + 1, 0);
 
     // the input range and the first launch's zero fills run on the side
     // stream, beside the first matching launch (which needs neither); the
@@ -986,15 +1035,18 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
             const int run = std::min(tnew_hi - t + 1, ring - slot);
             BM3D_HIP(bm3d::launch_zero(p->corner + (size_t)slot * corner_target, (size_t)run * corner_target, 0, 1,
                                        zs));
+            BM3D_HIP(bm3d::launch_zero(reinterpret_cast<float*>(p->wrap) + (size_t)slot * corner_target,
+                                       (size_t)run * corner_target, 0, 1, zs, p->wrap_flag));
             t += run;
         }
         for (int t = tnew_lo; t <= tnew_hi; ++t) {
             const int clo = contributors_lo(t, r), chi = contributors_hi(t, r, N);
             const int k = (clo - b0) / chunk;
             const bool whole = clo >= b0 && chi < b1 && chi < b0 + (k + 1) * chunk;
+            partial[t - tlo_all] = !whole;
             if (!whole)
-                BM3D_HIP(bm3d::launch_zero(reinterpret_cast<float*>(p->scratch) + (size_t)(t % ring) * p->region_ints,
-                                           (size_t)p->region_ints, 0, 1, zs));
+                BM3D_HIP(bm3d::launch_zero(reinterpret_cast<float*>(p->part + (size_t)(t % ring) * corner_target),
+                                           (size_t)corner_target * 2, 0, 1, zs));
         }
         touched_hi = std::max(touched_hi, tnew_hi);
         const bool joined = first_launch && side != stream;
@@ -1011,11 +1063,15 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
         // outputs whose evaluations are all done
         const int out_hi = (b1 - 1 == N - 1) ? first_frame + count - 1
                                               : std::min(first_frame + count - 1, b1 - 1 - r);
-        if (out_hi >= next_out) {
+        while (out_hi >= next_out) {
+            const int n = std::min(out_hi - next_out + 1, bm3d::kVaggMaxFrames);
             va.first_frame = next_out;
             va.dst = dst + (size_t)(next_out - first_frame) * dst_frame_stride;
-            BM3D_HIP(bm3d::launch_vagg_targets(va, out_hi - next_out + 1, stream));
-            next_out = out_hi + 1;
+            std::memset(va.partial, 0, sizeof va.partial);
+            for (int f = 0; f < n; ++f)
+                if (partial[next_out + f - tlo_all]) va.partial[f >> 5] |= 1u << (f & 31);
+            BM3D_HIP(bm3d::launch_vagg_targets(va, n, stream));
+            next_out += n;
         }
         b0 = b1;
     }
@@ -1024,6 +1080,7 @@ int bm3d_execute_vagg(bm3d_plan* p, float* dst, long long dst_frame_stride, long
 }
 
 int bm3d_execute_window(bm3d_plan* p, float* d_res, const float* d_src, void* stream) {
+    g_last_error.clear();
     if (!p || !d_res || !d_src) return fail(BM3D_EINVAL, "BM3D: null argument");
     const long long fs = p->plane_size;           // one slot
     const long long ps = (long long)p->tw * fs;   // one plane of slots

@@ -78,14 +78,79 @@ __device__ __forceinline__ int dpp_mov(int v) {
 // Exponent of the fixed-point unit of the accumulation regions. With
 // range null, fx_exp (extractor_exp - 24: pre-rounded values are whole
 // multiples of the unit). Otherwise fx_exp (-22) plus floor(log2 M) for
-// the largest finite |sample| M >= 2 of the call's input: a region pixel's
-// sum of weighted estimates (weights <= 1, a few hundred members at most)
-// then has the int32 headroom of 2^31 units >= 256 M whatever the scale of
-// the samples, and every input within (-2, 2) keeps the unit 2^-22.
+// the largest finite |sample| M >= 2 of the call's input, so the unit keeps
+// 22 bits below the samples' scale whatever that scale is, and every input
+// within (-2, 2) keeps the unit 2^-22. A pixel's sum may still pass 2^31
+// units (many members of weight near 1 on flat content); the region words
+// then wrap, and the wraps are counted exactly (add_wrap below).
 __device__ __forceinline__ int fx_exponent(const float* range, int fx_exp) {
     if (!range) return fx_exp;
     const float m = *range;
     return m >= 2.0f ? fx_exp + ilogbf(m) : fx_exp;
+}
+
+// ---- int32 region words with exact wrap accounting ----------------------
+// A region word holds the low 32 bits (two's complement) of a pixel's sum of
+// fixed-point units. Every add that carries the word past INT_MAX or below
+// INT_MIN is counted, +1 or -1, in a 32-bit wrap map with the layout of the
+// corner weight map, and readers add count * 2^32: the sum is exact for any
+// magnitude. Adds only check the returned old value against 2^30 (two VALU
+// operations per add) while every added value is below 2^30 in magnitude;
+// the exact wrap is computed only when that check fires.
+
+// Wrap of old + v: +1 (carried past INT_MAX), -1 (below INT_MIN), else 0.
+__device__ __forceinline__ int add_wrap(int old, int v) {
+    const int s = (int)((unsigned)old + (unsigned)v);
+    return ((old ^ s) & (v ^ s)) < 0 ? (v < 0 ? -1 : 1) : 0;
+}
+// Bit 31 set iff |old| >= 2^30, when an add of |v| < 2^30 can wrap.
+__device__ __forceinline__ unsigned wrap_risk(int old) { return (unsigned)old + 0x40000000u; }
+
+// Where wraps go: the count map (corner-map layout) and a flag that readers
+// test before reading it (set once any wrap was counted).
+struct WrapSink {
+    int* map;
+    int* flag;
+    __device__ __forceinline__ void count(size_t idx, int c) const {
+        atomicAdd(map + idx, c);
+        atomicOr(flag, 1);
+    }
+};
+
+// Adds whole-number units x of any magnitude below 2^62 to region word q:
+// its low 32 bits go to the word, the rest and the add's own wrap to the map.
+__device__ __forceinline__ void add_units_wide(int* q, float x, const WrapSink& ws, size_t idx) {
+    const long long xi = (long long)x;
+    const int lo = (int)(unsigned)(unsigned long long)xi;
+    const int hi = (int)((xi - (long long)lo) >> 32);
+    const int old = atomicAdd(q, lo);
+    const int c = hi + add_wrap(old, lo);
+    if (c) ws.count(idx, c);
+}
+
+// Adds values iv[j][i] (|iv| < 2^30) to words q + j * qrow + i, whose
+// pixels are map indices idx0 + j * irow + i, counting wraps exactly.
+template <int kR, int kC>
+__device__ __forceinline__ void add_units(int* q, int qrow, const int (&iv)[kR][kC], const WrapSink& ws,
+                                          size_t idx0, size_t irow) {
+    int old[kR][kC];
+    unsigned risk = 0;
+#pragma unroll
+    for (int j = 0; j < kR; ++j)
+#pragma unroll
+        for (int i = 0; i < kC; ++i) {
+            old[j][i] = atomicAdd(q + j * qrow + i, iv[j][i]);
+            risk |= wrap_risk(old[j][i]);
+        }
+    if (__builtin_expect((int)risk < 0, 0)) {
+#pragma unroll
+        for (int j = 0; j < kR; ++j)
+#pragma unroll
+            for (int i = 0; i < kC; ++i) {
+                const int c = add_wrap(old[j][i], iv[j][i]);
+                if (c) ws.count(idx0 + j * irow + i, c);
+            }
+    }
 }
 
 // xor1 / xor2 are quad permutations; xor4 is a half-row mirror followed by a
@@ -161,6 +226,8 @@ __device__ __forceinline__ void filter_walk(const FilterArgs& a, Wg w, int* acc,
     const int r = a.radius, tw = 2 * r + 1, N = a.num_frames;
     const int regsize = a.rh * a.rw;
     const int tx = w.x, ty = w.y;
+    const int rx0 = min(tx * a.tbx * a.bs, a.W - 8) - a.rm;  // region origin in the plane
+    const int ry0 = min(ty * a.tby * a.bs, a.H - 8) - a.rm;
     int e0, e1;
     if (a.mode == 0) {
         e0 = a.first_frame + w.z;
@@ -200,11 +267,28 @@ __device__ __forceinline__ void filter_walk(const FilterArgs& a, Wg w, int* acc,
                     int* src = acc + (t % tw) * regsize;
                     int* dst = a.scratch + (size_t)(t % a.ring) * a.target_stride +
                                (((size_t)ty * a.ntx + tx) * nplanes + p) * regsize;
-                    for (int i = threadIdx.x; i < regsize; i += blockDim.x) {
-                        const int v = src[i];
-                        if (whole) dst[i] = v;
-                        else if (v != 0) atomicAdd(dst + i, v);
-                        src[i] = 0;
+                    if (whole) {
+                        for (int i = threadIdx.x; i < regsize; i += blockDim.x) {
+                            dst[i] = src[i];
+                            src[i] = 0;
+                        }
+                    } else {
+                        // a frame whose contributors span several runs: its
+                        // region adds to a 64-bit per-pixel sum (zeroed by the
+                        // caller), which cannot wrap
+                        unsigned long long* q = reinterpret_cast<unsigned long long*>(a.part) +
+                                                (size_t)(t % a.ring) * a.corner_frame_stride +
+                                                (size_t)p * a.plane_size;
+                        for (int i = threadIdx.x; i < regsize; i += blockDim.x) {
+                            const int v = src[i];
+                            if (v != 0) {
+                                // region pixel (lx, ly) is plane pixel (rx0 + lx, ry0 + ly)
+                                const int ly = i / a.rw, lx = i - ly * a.rw;
+                                atomicAdd(q + (size_t)(ry0 + ly) * a.stride + rx0 + lx,
+                                          (unsigned long long)(long long)v);
+                            }
+                            src[i] = 0;
+                        }
                     }
                 }
                 flushed = max(flushed, hi);

@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "bm3d_device.h"
@@ -152,13 +154,6 @@ __device__ __forceinline__ void forward_mx_rows(f4 (&z)[4], float& mu, float& s,
     }
 }
 
-__device__ __forceinline__ void forward_mx(f4 (&z)[4], float& mu, float& s, const MxOps& o,
-                                           const float* __restrict__ mp, int stride, int q, int c) {
-    f4u v[4];
-    load_rows(v, mp, stride, q);
-    forward_mx_rows(z, mu, s, o, v, c);
-}
-
 // Inverse 3-D transform of shrunk coefficients z (layout of forward_mx):
 // x[T][i] = estimate at pixel (2 T + q / 2, 4 (q % 2) + i) of block c,
 // scaled by 4096 (the transform pair's gain).
@@ -188,7 +183,7 @@ __device__ __forceinline__ float row_sum16(float s) {
 // Grid (ntx, nty, grid_z), blockDim.x = 64 x waves. Same tile, region and
 // scratch layout, and the same accumulation modes, as bm3d_filter
 // (bm3d_kernels.hip, filter_walk).
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kWide>
 __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     extern __shared__ float lds[];
     __shared__ SlotTable st;
@@ -203,6 +198,13 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
     const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
+    const WrapSink wsink{a.wrap, a.wrap_flag};
+    // When one weighted estimate can reach 2^30 units (a small extractor_exp
+    // on large samples; 5/4 covers the FP16 transforms' rounding), every
+    // value takes the split 64-bit add: the kWide variant of the kernel,
+    // launched beside the other, does the work then and the other returns.
+    const bool wide = (1.25f * a.vbound * *a.range + E) * fx_scale >= 0x1p30f;
+    if (wide != kWide) return;  // uniform over the work-group
 
     MxOps o;
     o.load(reinterpret_cast<const uint4*>(a.mx_ops), lane);
@@ -279,7 +281,9 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
             } else {
                 // hard thresholding (source/kernel.cu:208-252); the DC
                 // coefficient is compared with 0 instead of sigma
-                forward_mx(zc, mu, sc, o, mp, a.stride, q, c);
+                f4u v[4];
+                load_rows(v, mp, a.stride, q);
+                forward_mx_rows(zc, mu, sc, o, v, c);
                 const float thr_ac = sigma / sc;  // sigma in units of the scaled residual
                 int kept0 = 0, kept1 = 0;
 #pragma unroll
@@ -309,8 +313,23 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
                 const float wm = w * mu;
                 const int lx = px - rx0, ly = py - ry0;
                 const int r = q >> 1, col = 4 * (q & 1);
-                if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
+                if (kWide && lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
+                    // split 64-bit adds (one estimate can reach 2^30 units)
                     int* qa = acc + st.region[z] + (ly + r) * a.rw + lx + col;
+                    const size_t widx = st.corner[z] + (size_t)(py + r) * a.stride + px + col;
+                    for (int k = 0; k < mult; ++k)
+#pragma unroll
+                        for (int T = 0; T < 4; ++T)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                float val = fma_(ws, x[T][i], wm);
+                                val = (val + E) - E;
+                                add_units_wide(qa + 2 * T * a.rw + i, __builtin_rintf(val * fx_scale), wsink,
+                                               widx + (size_t)(2 * T) * a.stride + i);
+                            }
+                } else if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
+                    int* qa = acc + st.region[z] + (ly + r) * a.rw + lx + col;
+                    const size_t widx = st.corner[z] + (size_t)(py + r) * a.stride + px + col;
                     int iv[4][4];
                     if (E == 0.0f) {
                         // no pre-rounding: the unit's power-of-two scale folds
@@ -331,12 +350,7 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
                             }
                     }
                     // a slot read `mult` times by VAggregate is added as often
-                    for (int k = 0; k < mult; ++k) {
-#pragma unroll
-                        for (int T = 0; T < 4; ++T)
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) atomicAdd(qa + 2 * T * a.rw + i, iv[T][i]);
-                    }
+                    for (int k = 0; k < mult; ++k) add_units<4, 4>(qa, 2 * a.rw, iv, wsink, widx, 2 * (size_t)a.stride);
                 } else if (res) {
                     float* qr = res + (size_t)(2 * z) * a.plane_size + (size_t)(py + r) * a.stride + px + col;
 #pragma unroll
@@ -358,17 +372,17 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     });
 }
 
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kWide>
 __global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
-    filter_mx_body<kFinal, kChroma>(a, launch_wg());
+    filter_mx_body<kFinal, kChroma, kWide>(a, launch_wg());
 }
 
-template <bool kFinal, bool kChroma>
+template <bool kFinal, bool kChroma, bool kWide>
 __global__ __launch_bounds__(512) void bm3d_filter_mx_multi(const FilterArgs* __restrict__ jobs,
                                                             const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const FilterArgs& a = jobs[j];
-    filter_mx_body<kFinal, kChroma>(a, job_wg(grids[j], blockIdx.x));
+    filter_mx_body<kFinal, kChroma, kWide>(a, job_wg(grids[j], blockIdx.x));
 }
 
 }  // namespace
@@ -424,52 +438,112 @@ void build_filter_mx_ops(uint32_t* out, int t2d, int t1d) {
     }
 }
 
+// Per axis with forward rows F[k][.] and inverse columns I[.][k]: a
+// coefficient k of samples bounded by M is at most ||F_k||_1 M. Hard
+// thresholding: |w est(q)| = |sum over kept k of c_k I(q, k)| / (4096 N_kept)
+// <= max_k |c_k I(q, k)| / 4096, a product over the three axes of
+// max_k ||F_k||_1 max_q |I(q, k)|. Wiener: w = 1 / sum coef^2 with the DC
+// coefficient 1, so by Cauchy-Schwarz |w est(q)| <= sqrt(sum_k c_k^2
+// I(q, k)^2) / 4096, a product over the axes of max_q sqrt(sum_k
+// (||F_k||_1 I(q, k))^2).
+float contribution_bound(int t2d, int t1d, bool final_) {
+    auto axis = [&](int kind) {
+        double hard = 0.0, wien = 0.0;
+        double l1[8];
+        for (int k = 0; k < 8; ++k) {
+            l1[k] = 0.0;
+            for (int x = 0; x < 8; ++x) l1[k] += ::fabs(xform_entry(kind, 0, k, x));
+        }
+        for (int k = 0; k < 8; ++k)
+            for (int q = 0; q < 8; ++q) hard = std::max(hard, l1[k] * ::fabs(xform_entry(kind, 1, q, k)));
+        for (int q = 0; q < 8; ++q) {
+            double s = 0.0;
+            for (int k = 0; k < 8; ++k) {
+                const double t = l1[k] * xform_entry(kind, 1, q, k);
+                s += t * t;
+            }
+            wien = std::max(wien, ::sqrt(s));
+        }
+        return final_ ? wien : hard;
+    };
+    const double a2 = axis(t2d), a1 = axis(t1d);
+    return (float)(a2 * a2 * a1 / 4096.0 * 1.0001);
+}
+
+template <bool kWide>
+void launch_filter_mx_kind(const FilterArgs& a, bool final_, bool chroma, dim3 grid, dim3 block, size_t lds_bytes,
+                           hipStream_t s) {
+    if (final_) {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<true, true, kWide>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter_mx<true, false, kWide>), grid, block, lds_bytes, s, a);
+    } else {
+        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<false, true, kWide>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_filter_mx<false, false, kWide>), grid, block, lds_bytes, s, a);
+    }
+}
+
+// Without extractor_exp one weighted estimate stays below 2^28 units (the
+// unit keeps 22 bits below the samples' scale and vbound <= 23), so only
+// plans with it can need the kWide variant; both are launched then and one
+// of them returns at once.
 hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                             hipStream_t s) {
     dim3 grid(a.ntx, a.nty, grid_z), block(64 * waves);
-    if (final_) {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<true, true>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_filter_mx<true, false>), grid, block, lds_bytes, s, a);
-    } else {
-        if (chroma) hipLaunchKernelGGL((bm3d_filter_mx<false, true>), grid, block, lds_bytes, s, a);
-        else hipLaunchKernelGGL((bm3d_filter_mx<false, false>), grid, block, lds_bytes, s, a);
-    }
+    launch_filter_mx_kind<false>(a, final_, chroma, grid, block, lds_bytes, s);
+    if (a.extractor != 0.0f) launch_filter_mx_kind<true>(a, final_, chroma, grid, block, lds_bytes, s);
     return hipGetLastError();
+}
+
+template <bool kWide>
+void launch_filter_mx_multi_kind(const FilterArgs* jobs, const JobGrid* grids, int njobs, dim3 grid, dim3 block,
+                                 bool final_, bool chroma, size_t lds_bytes, hipStream_t s) {
+    if (final_) {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, true, kWide>), grid, block, lds_bytes, s, jobs, grids, njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, false, kWide>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+    } else {
+        if (chroma)
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, true, kWide>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+        else
+            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, false, kWide>), grid, block, lds_bytes, s, jobs, grids,
+                               njobs);
+    }
 }
 
 hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
-                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s) {
+                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s, bool any_extractor) {
     dim3 grid(wgs), block(64 * waves);
-    if (final_) {
-        if (chroma)
-            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
-        else
-            hipLaunchKernelGGL((bm3d_filter_mx_multi<true, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
-    } else {
-        if (chroma)
-            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, true>), grid, block, lds_bytes, s, jobs, grids, njobs);
-        else
-            hipLaunchKernelGGL((bm3d_filter_mx_multi<false, false>), grid, block, lds_bytes, s, jobs, grids, njobs);
-    }
+    launch_filter_mx_multi_kind<false>(jobs, grids, njobs, grid, block, final_, chroma, lds_bytes, s);
+    if (any_extractor) launch_filter_mx_multi_kind<true>(jobs, grids, njobs, grid, block, final_, chroma, lds_bytes, s);
     return hipGetLastError();
 }
 
-hipError_t set_filter_mx_lds_limit(size_t bytes) {
+template <bool kWide>
+hipError_t set_filter_mx_kind_lds_limit(size_t bytes) {
     const void* ks[] = {
-        reinterpret_cast<const void*>(&bm3d_filter_mx<false, false>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx<false, true>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx<true, false>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx<true, true>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, false>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, true>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, false>),
-        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, true>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<false, false, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<false, true, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<true, false, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx<true, true, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, false, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<false, true, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, false, kWide>),
+        reinterpret_cast<const void*>(&bm3d_filter_mx_multi<true, true, kWide>),
     };
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t set_filter_mx_lds_limit(size_t bytes) {
+    hipError_t e = set_filter_mx_kind_lds_limit<false>(bytes);
+    if (e == hipSuccess) e = set_filter_mx_kind_lds_limit<true>(bytes);
+    return e;
 }
 
 }  // namespace bm3d

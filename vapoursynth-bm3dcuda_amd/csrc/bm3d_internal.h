@@ -49,6 +49,21 @@ struct bm3d_plan {
     // input's range (measured into *range per call) unless extractor_exp is set
     int fx_exp = 0;
     float* range = nullptr;
+    // wrap counts of the int32 region words (corner-map layout, corner_cap
+    // entries) and the flag readers test before reading them: the int just
+    // after *range, zero at creation and set for good once a wrap is counted
+    // (from then on the counts are zeroed with the corner maps)
+    int* wrap = nullptr;
+    int* wrap_flag = nullptr;
+    // bm3d_execute_vagg: 64-bit per-pixel sums of output frames whose
+    // contributing evaluations span several runs (corner-map layout)
+    long long* part = nullptr;
+    size_t part_cap = 0;
+    // |one weighted estimate| <= vbound * (largest |sample|)
+    float vbound = 0.0f;
+    // set by a call made while its stream was capturing: the captured graph
+    // holds the plan's buffers and tiles, so they may no longer change
+    bool pinned = false;
     // mixed-precision matching kernel (precision MIXED)
     bool use_dx = false, dx_ok = false;
     int tile_hp2 = 0;

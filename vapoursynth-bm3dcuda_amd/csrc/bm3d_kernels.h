@@ -73,6 +73,10 @@ struct FilterArgs {
     int* scratch;            // fixed-point regions (layout by mode, below)
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
+    int* wrap;               // wrap counts of the region words, corner-map layout (bm3d_device.h)
+    int* wrap_flag;          // set when any wrap was counted
+    const float* range;      // largest finite |sample| of the call's input (always measured)
+    float vbound;            // |one weighted estimate| <= vbound * range (contribution_bound)
     const void* mx_ops;      // mixed filter: per-lane transform operands (build_filter_mx_ops)
     int t2d, t1d;            // FP32 filter: transform kinds along rows / columns and members
     // Accumulation mode.
@@ -89,9 +93,12 @@ struct FilterArgs {
     //    [nty][ntx][planes][rh][rw]; weights at corner + (t % ring) *
     //    corner_frame_stride + [planes][H][stride]. A region whose
     //    contributing evaluations all lie in the work-group's chunk is
-    //    stored, any other is added atomically (the caller zeroes it first).
-    //    Group lists of evaluation m at index m - eval_lo.
+    //    stored; the regions of any other frame are added per pixel to
+    //    part + (t % ring) * corner_frame_stride + [planes][H][stride]
+    //    (64-bit, zeroed by the caller). Group lists of evaluation m at
+    //    index m - eval_lo.
     int mode;
+    long long* part;
     int eval_lo, eval_hi, chunk;
     int ring;
     long long target_stride;
@@ -101,6 +108,9 @@ struct FinalizeArgs {
     const int* scratch;
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
+    const int* wrap;         // wrap counts, corner-map layout; read when *wrap_flag
+    const int* wrap_flag;
+    long long corner_frame_stride;
     float* res;
     long long res_frame_stride, res_plane_stride;
     int W, H, stride;
@@ -133,8 +143,10 @@ struct DivideArgs {
     int nplanes;
 };
 
-// Output rows per work-group of bm3d_vagg_targets (64 columns wide).
+// Output rows per work-group of bm3d_vagg_targets (64 columns wide), and
+// output frames per launch (VTargetArgs::partial).
 constexpr int kVaggRows = 32;
+constexpr int kVaggMaxFrames = 256;
 
 // VAggregate from target-mode accumulation (bm3d_vagg_targets): output
 // frame n = the regions' integer sum of frame n (times the unit) over the
@@ -147,6 +159,13 @@ struct VTargetArgs {
     int ring;                // frame n at index n % ring
     const float* fx_range;   // fixed-point unit: see fx_exponent (bm3d_device.h)
     int fx_exp;
+    const int* wrap;         // wrap counts, corner-map layout; read when *wrap_flag
+    const int* wrap_flag;
+    // frames first_frame + f whose contributors spanned several runs (bit f,
+    // f < 256): their sums are the 64-bit per-pixel values at part + (frame %
+    // ring) * corner_frame + [planes][H][stride], not the regions
+    const long long* part;
+    unsigned partial[8];
     float* dst;              // output frames [first_frame, first_frame + count)
     long long dst_frame_stride, dst_plane_stride;
     int dst_stride;
@@ -186,8 +205,10 @@ hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, in
 // generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);
+// any_extractor: some job has extractor_exp set (launches the 64-bit-add
+// variant beside the other; bm3d_filter_mx.hip).
 hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
-                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s);
+                                  bool chroma, int waves, size_t lds_bytes, hipStream_t s, bool any_extractor);
 hipError_t launch_vagg_targets_multi(const VTargetArgs* jobs, const JobGrid* grids, int njobs, int wgs,
                                      hipStream_t s);
 // Largest |sample| per job: grid of job j = (min(H, 64), frames * nplanes).
@@ -226,8 +247,13 @@ hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int g
                             hipStream_t s);
 hipError_t set_filter_mx_lds_limit(size_t bytes);
 hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s);
-// Zeroes `count` runs of `n` floats, `step` floats apart.
-hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s);
+// Zeroes `count` runs of `n` floats, `step` floats apart; when `flag` is
+// given, only if *flag is non-zero (read on the device).
+hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s, const int* flag = nullptr);
+// Bound B on one weighted estimate of a group, |w * est| <= B * M for inputs
+// of magnitude <= M (hard thresholding, or Wiener when final), for the
+// transform pair (t2d, t1d); host.
+float contribution_bound(int t2d, int t1d, bool final_);
 // Largest finite |sample| of planes 0 .. nplanes-1 of `frames` frames from
 // clip, max-ed into *out (bit pattern compare; *out must start at zero).
 hipError_t launch_absmax(const float* clip, long long frame_stride, long long plane_stride, int nplanes, int W,

@@ -376,6 +376,10 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
     const int ngroups = a.tbx * a.tby;
     const float E = a.extractor;
     const float fx_scale = ldexpf(1.0f, -fx_exponent(a.fx_range, a.fx_exp));
+    const WrapSink ws{a.wrap, a.wrap_flag};
+    // one weighted estimate can reach 2^30 units (tiny extractor_exp on
+    // large samples): every value then takes the split 64-bit add
+    const bool wide = (a.vbound * *a.range + E) * fx_scale >= 0x1p30f;
 
     filter_walk(a, wg, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
@@ -483,17 +487,26 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, Wg wg) {
                     const int lx = px - rx0, ly = py - ry0;
                     if (lx >= 0 && ly >= 0 && lx + 8 <= a.rw && ly + 8 <= a.rh) {
                         int* q = acc + st.region[z] + ly * a.rw + lx + k;
-                        int iv[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            float val = w * v[i][j];
-                            val = (val + E) - E;
-                            iv[j] = (int)__builtin_rintf(val * fx_scale);
-                        }
+                        const size_t widx = st.corner[z] + (size_t)py * a.stride + px + k;
                         // a slot read `mult` times by VAggregate is added as often
-                        for (int c = 0; c < mult; ++c) {
+                        if (!wide) {
+                            int iv[8][1];
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) atomicAdd(q + j * a.rw, iv[j]);
+                            for (int j = 0; j < 8; ++j) {
+                                float val = w * v[i][j];
+                                val = (val + E) - E;
+                                iv[j][0] = (int)__builtin_rintf(val * fx_scale);
+                            }
+                            for (int c = 0; c < mult; ++c) add_units<8, 1>(q, a.rw, iv, ws, widx, a.stride);
+                        } else {
+                            for (int c = 0; c < mult; ++c)
+#pragma unroll
+                                for (int j = 0; j < 8; ++j) {
+                                    float val = w * v[i][j];
+                                    val = (val + E) - E;
+                                    add_units_wide(q + j * a.rw, __builtin_rintf(val * fx_scale), ws,
+                                                   widx + (size_t)j * a.stride);
+                                }
                         }
                     } else if (res) {
                         float* q = res + (size_t)(2 * z) * a.plane_size + (size_t)py * a.stride + px + k;
@@ -533,7 +546,9 @@ __global__ __launch_bounds__(256) void bm3d_filter_multi(const FilterArgs* __res
 
 // Zeroes grid.y runs of n floats, `step` floats apart (a fill kernel rather
 // than hipMemsetAsync, so the sequence can be captured in a graph).
-__global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step) {
+// Zero fill; with `flag`, only when *flag is set (uniform early exit).
+__global__ void bm3d_zero(float* __restrict__ p, size_t n, size_t step, const int* __restrict__ flag) {
+    if (flag && *flag == 0) return;
     float* q = p + (size_t)blockIdx.y * step;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t inc = (size_t)gridDim.x * blockDim.x;
@@ -628,6 +643,10 @@ __global__ void bm3d_finalize(FinalizeArgs a) {
                 isum += reg[(z * a.rh + ly) * a.rw + lx];
             }
         }
+        if (*a.wrap_flag)  // uniform: some region word wrapped
+            isum += (long long)a.wrap[(size_t)f * a.corner_frame_stride + (size_t)(p * a.slots + z) * a.plane_size +
+                                      (size_t)y * a.stride + x]
+                    << 32;
     }
     num[0] = s + (float)isum * ldexpf(1.0f, fx_exponent(a.fx_range, a.fx_exp));
 }
@@ -785,6 +804,10 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
     const int region = a.rh * a.rw;
     const size_t tstride = (size_t)a.nplanes * region;  // ints between consecutive tiles
     const int* scr = a.scratch + (size_t)slot * a.target_stride + (size_t)p * region;
+    const int* wmap = *a.wrap_flag ? a.wrap + (size_t)slot * a.corner_frame + (size_t)p * a.plane_size : nullptr;
+    const long long* pmap = ((a.partial[fo >> 5] >> (fo & 31)) & 1u)
+                                ? a.part + (size_t)slot * a.corner_frame + (size_t)p * a.plane_size
+                                : nullptr;
     if (threadIdx.x < kR) {  // once per row, not once per lane
         int ylo, yhi;
         covering_tiles(min(y0 + (int)threadIdx.x, a.H - 1), th, 1.0f / (float)th, a.H - 8, a.rm, a.rh, a.nty, ylo,
@@ -810,7 +833,9 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
 #pragma unroll
         for (int aa = 0; aa < 8; ++aa) dv = dv + rowsum[jj + 7 - aa][tx];
         long long isum = 0;
-        if (xhi - xlo <= 1 && yhi - ylo <= 1) {
+        if (pmap) {
+            isum = pmap[(size_t)yc * a.stride + xc];
+        } else if (xhi - xlo <= 1 && yhi - ylo <= 1) {
             // at most 2 x 2 covering regions: four independent, predicated loads
             const int lya = yc - (min(ylo * th, a.H - 8) - a.rm);
             const int lyb = yc - (min((ylo + 1) * th, a.H - 8) - a.rm);
@@ -836,6 +861,7 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
                 ly -= min((tyy + 1) * th, a.H - 8) - min(tyy * th, a.H - 8);
             }
         }
+        if (wmap) isum += (long long)wmap[(size_t)yc * a.stride + xc] << 32;  // counted wraps
         const float num = 0.0f + (float)isum * fx_unit;
         if (y < a.H && x < a.W)
             a.dst[(size_t)fo * a.dst_frame_stride + (size_t)p * a.dst_plane_stride + (size_t)y * a.dst_stride + x] =
@@ -987,11 +1013,11 @@ hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s) {
+hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s, const int* flag) {
     if (n == 0 || count == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(bm3d_zero, dim3((unsigned)blocks, count), dim3(256), 0, s, p, n, step);
+    hipLaunchKernelGGL(bm3d_zero, dim3((unsigned)blocks, count), dim3(256), 0, s, p, n, step, flag);
     return hipGetLastError();
 }
 
