@@ -208,32 +208,20 @@ def oracle_crop_errors(core, noisy, clean):
     return out
 
 
-BATCH_SIZES = [(360, 640), (720, 1280), (1080, 1920), (2160, 3840)]
-BATCH_PARAMS = dict(block_step=4, bm_range=9, ps_num=2, ps_range=4)
-
-
 def batch_workload(n_clips=64, frames=8):
-    """The 64-clip serving mix: clip k draws its resolution from 360p..2160p,
-    radius from {0, 1, 2} and sigma from U[5, 25] with seed k; content and
-    integer motion as the main workload, noise sd = sigma / 255."""
-    import numpy as np
+    """The 64-clip serving mix (bm3d_amd.synth.batch_clips): clip k draws its
+    resolution from 360p..2160p, radius from {0, 1, 2} and sigma from
+    U[5, 25] with seed k; content and integer motion as the main workload,
+    noise sd = sigma / 255."""
     from bm3d_amd import synth
-    clips = []
-    for k in range(n_clips):
-        rng = np.random.default_rng(k)
-        h, w = BATCH_SIZES[int(rng.integers(0, 4))]
-        radius = int(rng.integers(0, 3))
-        sigma = float(rng.uniform(5.0, 25.0))
-        _, noisy = synth.make_clip(h, w, frames, 1, noise_sd=sigma, seed=k, motion=True)
-        clips.append(dict(noisy=noisy, radius=radius, sigma=sigma))
-    return clips
+    return synth.batch_clips(n_clips, frames)
 
 
 def run_batch(args):
     """Aggregate Mpixel/s of the 64-clip mix: one batched graph (mixed
     precision, and FP32) vs per-clip FP32 plan launches, two passes each."""
     import torch
-    from bm3d_amd import core
+    from bm3d_amd import core, synth
 
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
@@ -257,7 +245,7 @@ def run_batch(args):
     plans = []
     for c, s in zip(clips, srcs):
         _, _, h, w = s.shape
-        kw = dict(BATCH_PARAMS, radius=c["radius"], precision="fp32")
+        kw = dict(synth.BATCH_PARAMS, radius=c["radius"], precision="fp32")
         plans.append((core.Plan(w, h, c["sigma"], **kw), core.Plan(w, h, c["sigma"], final=True, **kw)))
 
     def per_clip():
@@ -271,7 +259,7 @@ def run_batch(args):
     del plans
     res = {"per_clip_fp32_mpixel_s": round(px / t_clip / 1e6, 1)}
     for prec in ("fp32", "mixed"):
-        jobs = [dict(BATCH_PARAMS, src=s, out=o, basic=b, passes=2, radius=c["radius"], sigma=[c["sigma"]],
+        jobs = [dict(synth.BATCH_PARAMS, src=s, out=o, basic=b, passes=2, radius=c["radius"], sigma=[c["sigma"]],
                      precision=prec) for c, s, o, b in zip(clips, srcs, outs, basics)]
         batch = core.Batch(jobs)
         if not args.no_autotune:

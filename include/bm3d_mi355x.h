@@ -94,11 +94,10 @@ typedef struct bm3d_desc {
     int transform_2d;    /* BM3D_TRANSFORM_*: along block rows and columns */
     int transform_1d;    /* BM3D_TRANSFORM_*: along the group's members    */
     /* Precision MIXED: candidates kept per temporal frame for exact
-     * rescoring. 0 = the smallest of 4, 8, 16 that is at least 2 ps_num;
-     * else 4, 8 or 16, at least 2 ps_num. More keys: group lists closer to
-     * the FP32 path's, slower matching (8 at ps_num 2: the FP32 lists on a
-     * 1080p test clip, 14% more matching time). No counterpart in the
-     * reference. */
+     * rescoring. 0 = 8 (16 for ps_num above 4); else 4, 8 or 16, at least
+     * 2 ps_num. Fewer keys: faster matching, group lists further from the
+     * FP32 path's (4 at ps_num 2: 0.0014% of the lists of a 1080p test clip
+     * differ, not only at ties; 8: none). No counterpart in the reference. */
     int mixed_temporal_keys;
 } bm3d_desc;
 

@@ -3,11 +3,13 @@ its own plans: BM3Dv2 per pass through bm3d_execute_vagg, unprocessed planes
 copied from the source (source/source.cpp:911-989). With extractor_exp set
 the sums are exact, so the comparison is bit for bit -- across clip sizes,
 radii 0..2, one or two passes, FP32 and mixed precision, chroma, several
-waves (a tiny scratch budget) and repeated graph replays."""
+waves (a tiny scratch budget) and repeated graph replays. The FP32 jobs
+are also checked against the oracle's pipeline."""
 import numpy as np
 import pytest
 
 from bm3d_amd import synth
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +94,33 @@ def test_batch_equals_plans(budget):
             np.testing.assert_array_equal(bj["out"].cpu().numpy(), w_out, err_msg=f"job {k} replay {rep}")
             if w_basic is not None:
                 np.testing.assert_array_equal(bj["basic"].cpu().numpy(), w_basic, err_msg=f"job {k} basic")
+
+
+def test_batch_fp32_jobs_equal_oracle():
+    """The FP32 jobs of the batch (two passes at radius 1; chroma at radius 2
+    with an unprocessed plane; a clip passed through for sigma 0) against
+    the oracle's BM3Dv2 pipeline, bit for bit (extractor_exp set)."""
+    core = _core()
+    jobs = _make_jobs()
+    batch = core.Batch(jobs)
+    batch.execute()
+    torch.cuda.synchronize()
+    checked = 0
+    for k, (bj, (H, W, N, P, prm)) in enumerate(zip(batch.jobs, JOBS)):
+        if prm["precision"] != "fp32":
+            continue
+        noisy = jobs[k]["src"].cpu().numpy()
+        p = O.Params(sigma=tuple(prm["sigma"]), block_step=prm["block_step"], bm_range=prm["bm_range"],
+                     radius=prm["radius"], ps_num=prm.get("ps_num", 2), ps_range=prm.get("ps_range", 4),
+                     chroma=prm.get("chroma", False), extractor_exp=10)
+        if prm["passes"] == 2:
+            want_b, want = O.two_pass(noisy, p)
+            np.testing.assert_array_equal(bj["basic"].cpu().numpy(), want_b, err_msg=f"job {k} basic")
+        else:
+            want = O.bm3dv2(noisy, None, p)
+        np.testing.assert_array_equal(bj["out"].cpu().numpy(), want, err_msg=f"job {k}")
+        checked += 1
+    assert checked == 3
 
 
 def test_batch_without_extractor_close_to_plans():

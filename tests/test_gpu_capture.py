@@ -57,3 +57,35 @@ def test_execute_vagg_graph_replays_equal_eager(precision):
         assert torch.equal(b, want_b), rep
         assert torch.equal(f, want_f), rep
         assert torch.equal(other, want_b), rep
+
+
+def test_captured_plan_is_pinned():
+    """A call made while the stream captures pins the plan: the graph holds
+    its buffers and tiles, so later calls that would grow the buffers, and
+    tuning changes, fail with an error instead of moving them. Calls that
+    fit the buffers keep working, eager and replayed."""
+    core = _core()
+    from bm3d_amd import _capi as C
+    H, W, N = 40, 48, 6
+    _, noisy = synth.make_clip(H, W, N, 1, noise_sd=10.0, seed=97, motion=True)
+    src = torch.from_numpy(noisy).cuda()
+    p = core.Plan(W, H, 10.0, 4, 4, 1, 2, 2, extractor_exp=10)
+    want = p.execute_vagg(src, first=0, count=2)
+    torch.cuda.synchronize()
+    out = torch.empty_like(want)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        p.execute_vagg(src, first=0, count=2, out=out)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(out, want)
+    with pytest.raises(C.BM3DError, match="captured graph"):
+        p.set_tuning(filter_tile_x=4)
+    with pytest.raises(C.BM3DError, match="captured graph"):
+        p.autotune(src, first=0, count=2)
+    with pytest.raises(C.BM3DError, match="captured graph"):
+        p.execute_vagg(src)  # all 6 frames: larger buffers than the capture saw
+    again = p.execute_vagg(src, first=0, count=2)  # fits: still fine
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(again, want) and torch.equal(out, want)

@@ -13,7 +13,8 @@
 * YUV444 3840x2160, 16 frames, chroma, two-pass BM3Dv2 radius 2,
   block_step 4, bm_range 16, ps_num 2, ps_range 6, extractor_exp 8, noise
   20/255: the mixed path at full size is bitwise run-to-run deterministic
-  and within 0.05 dB of the FP32 path per plane; on a 5-frame 128x128 crop
+  and within 0.05 dB of the FP32 path per plane, and its group lists equal
+  the FP32 path's except among exactly tied candidates; on a 5-frame 128x128 crop
   clip it is within 0.05 dB of the FP64 oracle per plane (max-abs stated).
 """
 import numpy as np
@@ -94,6 +95,20 @@ def test_config_4k_yuv444_r2_chroma():
 
     mixed = two_pass("mixed", src)
     np.testing.assert_array_equal(two_pass("mixed", src), mixed)  # bitwise deterministic
+    # basic-pass group lists of three frames: the mixed matcher's equal the
+    # FP32 matcher's except among exactly tied candidates
+    from bm3d_amd import core
+    import mixed_ties
+    mk = dict(block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=6, chroma=True)
+    plan_m = core.Plan(W, H, [20.0] * 3, precision="mixed", **mk)
+    plan_32 = core.Plan(W, H, [20.0] * 3, **mk)
+    src_t = torch.from_numpy(noisy).cuda()
+    for n in (0, 7, 15):
+        gm = core.unpack_groups(plan_m.match(src_t, first=n, count=1))
+        g32 = core.unpack_groups(plan_32.match(src_t, first=n, count=1))
+        nd, nl = mixed_ties.tie_only(gm, g32, noisy, 2, 4, first=n)
+        print(f"frame {n}: {nd} of {nl} lists differ, all at exact ties")
+    del plan_m, plan_32, src_t
     fp32 = two_pass("fp32", src)
     pm, pf = _psnr_planes(mixed, clean), _psnr_planes(fp32, clean)
     pn = _psnr_planes(noisy, clean)

@@ -2,7 +2,8 @@
 a VapourSynth plugin holds them, staged through pinned buffers, pipelined in
 chunks, on up to 4 resource sets. Bar: bit-identical to the device route
 (bm3d_filter_get_frames) on the same frames -- the kernels are the same;
-extractor_exp is set so that aggregation order cannot matter."""
+extractor_exp is set so that aggregation order cannot matter -- and, for the
+two-pass BM3Dv2, bit-identical to the oracle's two-pass pipeline."""
 import ctypes
 import threading
 
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 from bm3d_amd import synth
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -39,6 +41,13 @@ def test_bm3dv2_two_pass_host_equals_device(N):
     f_h = vs.BM3Dv2(host, ref=b_h, **KW)
     np.testing.assert_array_equal(b_h.numpy(), b_d.numpy())
     np.testing.assert_array_equal(f_h.numpy(), f_d.numpy())
+    # and both equal the oracle's two-pass BM3Dv2 (exact sums with extractor_exp)
+    p = O.Params(sigma=tuple(KW["sigma"]), block_step=KW["block_step"], bm_range=KW["bm_range"],
+                 radius=KW["radius"], ps_num=KW["ps_num"], ps_range=KW["ps_range"],
+                 extractor_exp=KW["extractor_exp"])
+    want_b, want_f = O.two_pass(noisy, p)
+    np.testing.assert_array_equal(b_h.numpy(), want_b)
+    np.testing.assert_array_equal(f_h.numpy(), want_f)
 
 
 @pytest.mark.parametrize("radius", [0, 2])

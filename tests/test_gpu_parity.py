@@ -11,6 +11,7 @@ Bars:
 import numpy as np
 import pytest
 
+import mixed_ties
 from bm3d_amd import synth
 from oracle import oracle as O
 
@@ -354,28 +355,31 @@ def _mixed_input(kind, noisy):
     raise ValueError(kind)
 
 
-# minimum share of identical group lists per input kind
-MIXED_KINDS = {"noisy": 0.995, "quantised": 0.9, "flat": 1.0, "wide": 1.0, "offset": 0.995, "nan": 1.0}
+# input kinds; True: the lists must be identical (no ties, or the FP32 fallback)
+MIXED_KINDS = {"noisy": False, "quantised": False, "flat": True, "wide": True, "offset": False, "nan": True}
 
 
 @pytest.mark.parametrize("kind", list(MIXED_KINDS))
 @pytest.mark.parametrize("case", CASES, ids=_ids())
 def test_mixed_groups_agree_fp32(case, kind):
+    """Mixed lists equal the FP32 lists except among exactly tied candidates
+    (quantised and offset inputs have many exact ties)."""
     core = _gpu()
     H, W, N, P, chroma, prm = case
     _, noisy = synth.make_clip(H, W, N, P, noise_sd=prm["sigma"][0], seed=3, motion=True, yuv=chroma)
-    src = torch.from_numpy(_mixed_input(kind, noisy)).cuda()
+    x = _mixed_input(kind, noisy)
+    src = torch.from_numpy(x).cuda()
     p32 = _plan(core, W, H, prm, False, chroma)
     pmx = core.Plan(W, H, prm["sigma"], prm["block_step"], prm["bm_range"], prm["radius"],
                     prm["ps_num"], prm["ps_range"], chroma=chroma, precision="mixed")
     assert pmx.uses_mixed_kernel()
-    g32 = p32.match(src).cpu().numpy()
-    gmx = pmx.match(src).cpu().numpy()
-    same = float(np.all(gmx == g32, axis=-1).mean())
-    print(f"agreement {kind}: {same:.5f}")
-    assert same >= MIXED_KINDS[kind], f"{same:.4%} of group lists agree"
-    if MIXED_KINDS[kind] == 1.0:
+    g32 = core.unpack_groups(p32.match(src))
+    gmx = core.unpack_groups(pmx.match(src))
+    if MIXED_KINDS[kind]:
         np.testing.assert_array_equal(gmx, g32)
+    else:
+        nd, nl = mixed_ties.tie_only(gmx, g32, x, prm["radius"], prm["block_step"])
+        print(f"{kind}: {nd} of {nl} lists differ, all at exact ties")
     assert pmx.count_candidates(src) == p32.count_candidates(src)
 
 
@@ -390,7 +394,11 @@ def test_mixed_groups_agree_fp32(case, kind):
 def test_mixed_groups_agree_fp32_larger(prm):
     """Interior windows take the matcher's unchecked scan, border windows the
     checked one; block steps 3, 6 and 7 run its second kernel variant (block
-    step not dividing 8)."""
+    step not dividing 8). Bar: at most 0.01% of the lists differ from the
+    FP32 path's, and only among near-tied candidates (exact distances within
+    2^-7 of each other): the FP16 pre-score can rank such candidates either
+    way, and the kept keys are a margin, not a certificate. The bench
+    workload's lists are held to exact ties only (test_gpu_workloads.py)."""
     core = _gpu()
     H, W, N = 200, 264, 5
     _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=11, motion=True)
@@ -403,21 +411,24 @@ def test_mixed_groups_agree_fp32_larger(prm):
         if final:
             p32 = core.Plan(W, H, 15.0, final=True, **prm)
             pmx = core.Plan(W, H, 15.0, final=True, precision="mixed", **prm)
-            g32 = p32.match(src, basic).cpu().numpy()
-            gmx = pmx.match(src, basic).cpu().numpy()
+            g32 = core.unpack_groups(p32.match(src, basic))
+            gmx = core.unpack_groups(pmx.match(src, basic))
+            matched = basic.cpu().numpy()
         else:
-            g32 = p32.match(src).cpu().numpy()
-            gmx = pmx.match(src).cpu().numpy()
-        same = float(np.all(gmx == g32, axis=-1).mean())
-        print(f"agreement final={final}: {same:.5f}")
-        assert same >= 0.995
+            g32 = core.unpack_groups(p32.match(src))
+            gmx = core.unpack_groups(pmx.match(src))
+            matched = noisy
+        nd, nl = mixed_ties.tie_only(gmx, g32, matched, prm["radius"], prm["block_step"], rel=2.0 ** -7)
+        print(f"final={final}: {nd} of {nl} lists differ, all at near-ties")
+        assert nd <= 1e-4 * nl
 
 
 def test_mixed_temporal_keys_option():
     """desc.mixed_temporal_keys: keeping more candidates per temporal frame
     for exact rescoring keeps a superset, so every list that matched the FP32
-    path's still does; 0 means 4 at ps_num 2. Values other than 0, 4, 8, 16,
-    or below 2 ps_num, are rejected."""
+    path's still does; 0 means 8 at ps_num 2, whose lists differ from the
+    FP32 path's only at exact ties. Values other than 0, 4, 8, 16, or below
+    2 ps_num, are rejected."""
     core = _gpu()
     H, W, N = 200, 264, 5
     _, noisy = synth.make_clip(H, W, N, 1, noise_sd=15, seed=12, motion=True)
@@ -431,9 +442,10 @@ def test_mixed_temporal_keys_option():
         lists[kt] = pmx.match(src).cpu().numpy()
         agree[kt] = np.all(lists[kt] == g32, axis=-1)
     print({k: float(v.mean()) for k, v in agree.items()})
-    np.testing.assert_array_equal(lists[0], lists[4])
+    np.testing.assert_array_equal(lists[0], lists[8])
     assert not np.any(agree[4] & ~agree[8]) and not np.any(agree[8] & ~agree[16])
-    assert agree[8].mean() >= 0.9999
+    mixed_ties.tie_only(core.unpack_groups(torch.from_numpy(lists[8])), core.unpack_groups(torch.from_numpy(g32)),
+                        noisy, prm["radius"], prm["block_step"])
     for bad, pn in [(6, 2), (32, 2), (4, 3), (-4, 1)]:
         with pytest.raises(Exception, match="mixed_temporal_keys"):
             core.Plan(W, H, 15.0, precision="mixed", mixed_temporal_keys=bad, **dict(prm, ps_num=pn))

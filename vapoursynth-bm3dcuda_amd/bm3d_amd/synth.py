@@ -88,3 +88,29 @@ def sha256(a: np.ndarray) -> str:
 def psnr(a: np.ndarray, b: np.ndarray, peak: float = 1.0) -> float:
     mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
     return float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
+
+
+# The 64-clip serving mix: clip k draws its resolution, temporal radius and
+# sigma with seed k; content and integer motion as make_clip, noise sd =
+# sigma / 255. Every clip uses block_step 4, bm_range 9, ps_num 2, ps_range 4.
+BATCH_SIZES = [(360, 640), (720, 1280), (1080, 1920), (2160, 3840)]
+BATCH_PARAMS = dict(block_step=4, bm_range=9, ps_num=2, ps_range=4)
+
+
+def batch_clip_params(k: int):
+    """(height, width, radius, sigma) of clip k of the mix."""
+    rng = np.random.default_rng(k)
+    h, w = BATCH_SIZES[int(rng.integers(0, 4))]
+    radius = int(rng.integers(0, 3))
+    sigma = float(rng.uniform(5.0, 25.0))
+    return h, w, radius, sigma
+
+
+def batch_clips(n_clips: int = 64, frames: int = 8):
+    """The mix as dicts (noisy [frames, 1, H, W] float32, radius, sigma)."""
+    clips = []
+    for k in range(n_clips):
+        h, w, radius, sigma = batch_clip_params(k)
+        _, noisy = make_clip(h, w, frames, 1, noise_sd=sigma, seed=k, motion=True)
+        clips.append(dict(noisy=noisy, radius=radius, sigma=sigma))
+    return clips

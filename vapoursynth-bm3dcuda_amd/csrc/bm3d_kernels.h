@@ -224,8 +224,7 @@ hipError_t set_match_lds_limit(size_t bytes);
 // launch_match; LDS bytes as returned by dx_lds_bytes.
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_dx_lds_limit(size_t bytes);
-// Default keys kept per temporal frame: the smallest of 4, 8, 16 that is at
-// least 2 ps_num.
+// Default keys kept per temporal frame: 8 (16 for ps_num above 4).
 int dx_temporal_keys(int ps_num);
 // The mixed matcher has one kernel variant for block steps dividing 8 and
 // one for the others.

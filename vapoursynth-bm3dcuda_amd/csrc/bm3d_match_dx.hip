@@ -676,7 +676,7 @@ hipError_t launch_dx_multi_keys(const MatchArgs* jobs, const JobGrid* grids, int
 }
 }  // namespace
 
-int dx_temporal_keys(int ps_num) { return ps_num <= 2 ? 4 : (ps_num <= 4 ? 8 : 16); }
+int dx_temporal_keys(int ps_num) { return ps_num <= 4 ? 8 : 16; }
 
 bool dx_block_step_divides_8(int block_step) { return 8 % block_step == 0; }
 
