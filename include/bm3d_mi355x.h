@@ -148,6 +148,21 @@ int bm3d_execute_vagg(bm3d_plan *plan, float *dst, long long dst_frame_stride, l
                       int dst_stride, const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count,
                       void *stream);
 
+/* bm3d_execute without allocating device memory: no counterpart in the
+ * reference, for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it). Same arguments,
+ * checks and stacks as bm3d_execute, bit for bit when extractor_exp is set.
+ * The group lists, weight maps and accumulation regions live in a static
+ * 64 MiB device array that each GPU holds from the library's first use of
+ * it until the process ends, instead of the plan's buffers: one evaluation
+ * needs about 25 bytes per plane pixel and slot (a 1920x1080 grey plane at
+ * radius 0 fits, with block_step 4); plans that need more than 64 MiB for
+ * one evaluation get BM3D_EINVAL, naming the size. Every call on one GPU
+ * shares that array: calls that overlap on a GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+int bm3d_execute_noalloc(bm3d_plan *plan, float *d_res, long long res_frame_stride, long long res_plane_stride,
+                         const bm3d_clip *src, const bm3d_clip *ref, int first_frame, int count, void *stream);
+
 /* One window in the reference's packed input layout
  * d_src = [(final ? 2 : 1)][planes][2r+1][height][stride], ref first
  * (source/kernel.cu:43-44). Writes one packed stack to d_res. */

@@ -150,6 +150,9 @@ def lib():
     L.bm3d_execute_vagg.restype = c_int
     L.bm3d_execute_window.argtypes = [P, P, P, P]
     L.bm3d_execute_window.restype = c_int
+    L.bm3d_execute_noalloc.argtypes = [P, P, c_ll, c_ll, ctypes.POINTER(Clip), ctypes.POINTER(Clip),
+                                       c_int, c_int, P]
+    L.bm3d_execute_noalloc.restype = c_int
     L.bm3d_match.argtypes = [P, P, ctypes.POINTER(Clip), ctypes.POINTER(Clip), c_int, c_int, P]
     L.bm3d_match.restype = c_int
     L.bm3d_aggregate.argtypes = [P, c_ll, c_ll, c_int, P, c_ll, c_ll, c_int, c_int, c_int, c_int,
@@ -214,6 +217,7 @@ EXPORTS = (
     "bm3d_batch_autotune", "bm3d_batch_info", "bm3d_batch_destroy",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
     "bm3d_filter_get_frames", "bm3d_filter_destroy", "bm3d_filter_host_range", "bm3d_filter_get_frames_host",
+    "bm3d_execute_noalloc",
 )
 
 KERNEL_MATCH = 0

@@ -27,6 +27,13 @@ namespace {
 thread_local std::string g_last_error;
 }  // namespace
 
+// Device memory of bm3d_execute_noalloc: one array per GPU, held until the
+// process ends, shared by every call on that GPU (which is why such calls
+// must not overlap there). Host code uses its address from
+// hipGetSymbolAddress, never the name.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 int bm3d::fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
@@ -487,19 +494,37 @@ bm3d::VTargetArgs vtarget_args(const bm3d_plan* p, int ring) {
 
 // Slot-mode evaluation of frames n0 .. n0 + nf - 1 (stacks path): group
 // lists, slot regions and corner maps per evaluation.
-hipError_t evaluate_slots(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref, int n0, int nf, float* res,
-                          long long res_frame_stride, long long res_plane_stride, hipStream_t stream) {
+// The device buffers a slot-mode evaluation batch works in: the plan's own
+// (bm3d_execute), or carved from the static scratch (bm3d_execute_noalloc).
+struct SlotBufs {
+    uint32_t* groups;
+    float* corner;
+    int* wrap;
+    int* regions;
+    bool wrap_always_zero;  // zero the wrap counts every batch (else only once the plan counted a wrap)
+};
+
+SlotBufs plan_slot_bufs(const bm3d_plan* p) { return {p->groups, p->corner, p->wrap, p->scratch, false}; }
+
+hipError_t evaluate_slots(bm3d_plan* p, const SlotBufs& b, const bm3d_clip* src, const bm3d_clip* ref, int n0,
+                          int nf, float* res, long long res_frame_stride, long long res_plane_stride,
+                          hipStream_t stream) {
     const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
-    hipError_t e = bm3d::launch_zero(p->corner, (size_t)nf * corner_eval, 0, 1, stream);
+    hipError_t e = bm3d::launch_zero(b.corner, (size_t)nf * corner_eval, 0, 1, stream);
     if (e == hipSuccess)
-        e = bm3d::launch_zero(reinterpret_cast<float*>(p->wrap), (size_t)nf * corner_eval, 0, 1, stream, p->wrap_flag);
+        e = bm3d::launch_zero(reinterpret_cast<float*>(b.wrap), (size_t)nf * corner_eval, 0, 1, stream,
+                              b.wrap_always_zero ? nullptr : p->wrap_flag);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(0, stream);
-        e = launch_matching(p, p->groups, nullptr, src, ref, n0, nf, stream);
+        e = launch_matching(p, b.groups, nullptr, src, ref, n0, nf, stream);
     }
     if (e != hipSuccess) return e;
     bm3d::FilterArgs fa = filter_args(p, src, ref);
+    fa.groups = b.groups;
+    fa.corner = b.corner;
+    fa.wrap = b.wrap;
+    fa.scratch = b.regions;
     fa.mode = 0;
     fa.first_frame = n0;
     fa.res = res;
@@ -507,6 +532,110 @@ hipError_t evaluate_slots(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* r
     fa.res_plane_stride = res_plane_stride;
     fa.corner_frame_stride = corner_eval;
     return launch_filtering(p, fa, nf, stream);
+}
+
+// Checks of bm3d_execute's arguments; fills in packed strides.
+int check_execute(bm3d_plan* p, float* d_res, long long& res_frame_stride, long long& res_plane_stride,
+                  const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count) {
+    if (!p || !d_res || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
+    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
+    if (p->d.final_ && (ref->frame_stride != src->frame_stride || ref->plane_stride != src->plane_stride ||
+                        ref->num_frames != src->num_frames))
+        return fail(BM3D_EINVAL, "BM3D: \"ref\" must have the layout and number of frames of \"clip\"");
+    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
+        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
+    const long long plane_stack = (long long)p->tw * 2 * p->plane_size;
+    if (res_plane_stride == 0) res_plane_stride = plane_stack;
+    if (res_frame_stride == 0) res_frame_stride = p->nplanes * res_plane_stride;
+    if (res_plane_stride < plane_stack || res_frame_stride < (p->nplanes - 1) * res_plane_stride + plane_stack)
+        return fail(BM3D_EINVAL, "BM3D: result strides overlap");
+    int dev = 0;
+    BM3D_HIP(hipGetDevice(&dev));
+    if (dev != p->device) return fail(BM3D_EINVAL, "BM3D: plan used on a different device than it was created on");
+    return BM3D_OK;
+}
+
+// The stacks of output frames first_frame .. first_frame + count - 1, in
+// evaluation batches of `batch` frames over the buffers b.
+int run_slots(bm3d_plan* p, const SlotBufs& b, int batch, float* d_res, long long res_frame_stride,
+              long long res_plane_stride, const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
+              hipStream_t stream) {
+    const bm3d_desc& d = p->d;
+    const long long plane_stack = (long long)p->tw * 2 * p->plane_size;
+    const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
+    BM3D_HIP(measure_range(p, src, std::max(first_frame - d.radius, 0),
+                           std::min(first_frame + count - 1 + d.radius, src->num_frames - 1), stream));
+    for (int f0 = 0; f0 < count; f0 += batch) {
+        const int nf = std::min(batch, count - f0);
+        const int n0 = first_frame + f0;
+        float* res = d_res + (size_t)f0 * res_frame_stride;
+        // Stacks are fully written by finalize (sums) and den-box (weights);
+        // they start from zero only when members can land outside the LDS
+        // regions and go to global atomics.
+        const bool outliers = p->frm < p->margin;
+        if (outliers) {
+            if (res_plane_stride == plane_stack) {
+                BM3D_HIP(bm3d::launch_zero(res, (size_t)p->nplanes * plane_stack, (size_t)res_frame_stride, nf, stream));
+            } else {
+                for (int f = 0; f < nf; ++f)
+                    BM3D_HIP(bm3d::launch_zero(res + (size_t)f * res_frame_stride, (size_t)plane_stack,
+                                               (size_t)res_plane_stride, p->nplanes, stream));
+            }
+        }
+        BM3D_HIP(evaluate_slots(p, b, src, ref, n0, nf, res, res_frame_stride, res_plane_stride, stream));
+
+        bm3d::FinalizeArgs fz{};
+        fz.scratch = b.regions;
+        fz.fx_range = fx_range(p);
+        fz.fx_exp = p->fx_exp;
+        fz.wrap = b.wrap;
+        fz.wrap_flag = p->wrap_flag;
+        fz.corner_frame_stride = corner_eval;
+        fz.res = res;
+        fz.res_frame_stride = res_frame_stride;
+        fz.res_plane_stride = res_plane_stride;
+        fz.W = d.width; fz.H = d.height; fz.stride = d.stride;
+        fz.plane_size = p->plane_size;
+        fz.nplanes = p->nplanes;
+        fz.slots = p->tw;
+        fz.bs = d.block_step; fz.tbx = p->ftbx; fz.tby = p->ftby; fz.ntx = p->fntx; fz.nty = p->fnty;
+        fz.rm = p->frm; fz.rw = p->frw; fz.rh = p->frh;
+        fz.accumulate = outliers ? 1 : 0;
+        fz.chroma = d.chroma;
+        for (int i = 0; i < 3; ++i) fz.sigma[i] = p->sigma[i];
+        BM3D_HIP(bm3d::launch_finalize(fz, nf * p->nplanes * p->tw, stream));
+
+        bm3d::DenArgs da{};
+        da.corner = b.corner;
+        da.corner_frame_stride = corner_eval;
+        da.res = res;
+        da.res_frame_stride = res_frame_stride;
+        da.res_plane_stride = res_plane_stride;
+        da.W = d.width; da.H = d.height; da.stride = d.stride;
+        da.plane_size = p->plane_size;
+        da.nplanes = p->nplanes;
+        da.slots = p->tw;
+        BM3D_HIP(bm3d::launch_den_box(da, nf * p->nplanes * p->tw, stream));
+    }
+    return BM3D_OK;
+}
+
+// Address of the current device's `scratch` array (hipGetSymbolAddress once
+// per device, with that device current).
+int noalloc_scratch(void** out) {
+    static std::mutex mu;
+    static void* ptrs[64];
+    int dev = 0;
+    BM3D_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(BM3D_EUNSUPPORTED, "BM3D: bm3d_execute_noalloc serves devices 0..63");
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptrs[dev]) {
+        void* ptr = nullptr;
+        BM3D_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        ptrs[dev] = ptr;
+    }
+    *out = ptrs[dev];
+    return BM3D_OK;
 }
 
 // Evaluations that can add to output frame t in target mode:
@@ -650,90 +779,52 @@ void bm3d_plan_destroy(bm3d_plan* p) {
 
 long long bm3d_stack_size(const bm3d_plan* p) { return p ? p->stack_size : 0; }
 
+int bm3d_execute_noalloc(bm3d_plan* p, float* d_res, long long res_frame_stride, long long res_plane_stride,
+                         const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count, void* stream_) {
+    g_last_error.clear();
+    if (int rc = check_execute(p, d_res, res_frame_stride, res_plane_stride, src, ref, first_frame, count)) return rc;
+    if (count == 0) return BM3D_OK;
+    // one evaluation's buffers: group lists, corner weights, wrap counts, regions
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t corner_eval = (size_t)p->nplanes * p->tw * p->plane_size;
+    const size_t groups_b = up((size_t)p->nblocks * 8 * 4), corner_b = up(corner_eval * 4);
+    const size_t regions_b = up((size_t)p->tw * p->region_ints * 4);
+    const size_t per_eval = groups_b + 2 * corner_b + regions_b;
+    if (per_eval > kNoAllocScratchBytes)
+        return fail(BM3D_EINVAL, "BM3D: bm3d_execute_noalloc: one evaluation of this plan needs " +
+                                     std::to_string(per_eval) + " bytes of scratch, more than its " +
+                                     std::to_string(kNoAllocScratchBytes) + " (use bm3d_execute)");
+    void* base = nullptr;
+    if (int rc = noalloc_scratch(&base)) return rc;
+    const int batch = (int)std::min<size_t>((size_t)count, kNoAllocScratchBytes / per_eval);
+    char* c = static_cast<char*>(base);
+    SlotBufs b;
+    b.groups = reinterpret_cast<uint32_t*>(c);
+    c += groups_b * batch;
+    b.corner = reinterpret_cast<float*>(c);
+    c += corner_b * batch;
+    b.wrap = reinterpret_cast<int*>(c);
+    c += corner_b * batch;
+    b.regions = reinterpret_cast<int*>(c);
+    b.wrap_always_zero = true;  // the array keeps other calls' counts
+    return run_slots(p, b, batch, d_res, res_frame_stride, res_plane_stride, src, ref, first_frame, count,
+                     reinterpret_cast<hipStream_t>(stream_));
+}
+
 int bm3d_execute(bm3d_plan* p, float* d_res, long long res_frame_stride, long long res_plane_stride,
                  const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count, void* stream_) {
     g_last_error.clear();
-    if (!p || !d_res || !src || !src->data) return fail(BM3D_EINVAL, "BM3D: null argument");
-    if (p->d.final_ && (!ref || !ref->data)) return fail(BM3D_EINVAL, "BM3D: final pass needs \"ref\"");
-    if (p->d.final_ && (ref->frame_stride != src->frame_stride || ref->plane_stride != src->plane_stride ||
-                        ref->num_frames != src->num_frames))
-        return fail(BM3D_EINVAL, "BM3D: \"ref\" must have the layout and number of frames of \"clip\"");
-    if (src->num_frames <= 0 || count < 0 || first_frame < 0 || first_frame + count > src->num_frames)
-        return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
-    const long long plane_stack = (long long)p->tw * 2 * p->plane_size;
-    if (res_plane_stride == 0) res_plane_stride = plane_stack;
-    if (res_frame_stride == 0) res_frame_stride = p->nplanes * res_plane_stride;
-    if (res_plane_stride < plane_stack || res_frame_stride < (p->nplanes - 1) * res_plane_stride + plane_stack)
-        return fail(BM3D_EINVAL, "BM3D: result strides overlap");
+    if (int rc = check_execute(p, d_res, res_frame_stride, res_plane_stride, src, ref, first_frame, count)) return rc;
     if (count == 0) return BM3D_OK;
     hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
-    int dev = 0;
-    BM3D_HIP(hipGetDevice(&dev));
-    if (dev != p->device) return fail(BM3D_EINVAL, "BM3D: plan used on a different device than it was created on");
-
     const long long corner_eval = (long long)p->nplanes * p->tw * p->plane_size;
     const long long scratch_eval = (long long)p->tw * p->region_ints;
     const int batch = batch_limit(p, count, corner_eval * 8 + scratch_eval * 4 + (long long)p->nblocks * 32);
     if (int rc = ensure_buffers(p, (size_t)batch * p->nblocks * 8, (size_t)batch * corner_eval,
                                 (size_t)batch * scratch_eval, stream))
         return rc;
-
-    const bm3d_desc& d = p->d;
-    BM3D_HIP(measure_range(p, src, std::max(first_frame - d.radius, 0),
-                           std::min(first_frame + count - 1 + d.radius, src->num_frames - 1), stream));
-    for (int f0 = 0; f0 < count; f0 += batch) {
-        const int nf = std::min(batch, count - f0);
-        const int n0 = first_frame + f0;
-        float* res = d_res + (size_t)f0 * res_frame_stride;
-        // Stacks are fully written by finalize (sums) and den-box (weights);
-        // they start from zero only when members can land outside the LDS
-        // regions and go to global atomics.
-        const bool outliers = p->frm < p->margin;
-        if (outliers) {
-            if (res_plane_stride == plane_stack) {
-                BM3D_HIP(bm3d::launch_zero(res, (size_t)p->nplanes * plane_stack, (size_t)res_frame_stride, nf, stream));
-            } else {
-                for (int f = 0; f < nf; ++f)
-                    BM3D_HIP(bm3d::launch_zero(res + (size_t)f * res_frame_stride, (size_t)plane_stack,
-                                               (size_t)res_plane_stride, p->nplanes, stream));
-            }
-        }
-        BM3D_HIP(evaluate_slots(p, src, ref, n0, nf, res, res_frame_stride, res_plane_stride, stream));
-
-        bm3d::FinalizeArgs fz{};
-        fz.scratch = p->scratch;
-        fz.fx_range = fx_range(p);
-        fz.fx_exp = p->fx_exp;
-        fz.wrap = p->wrap;
-        fz.wrap_flag = p->wrap_flag;
-        fz.corner_frame_stride = corner_eval;
-        fz.res = res;
-        fz.res_frame_stride = res_frame_stride;
-        fz.res_plane_stride = res_plane_stride;
-        fz.W = d.width; fz.H = d.height; fz.stride = d.stride;
-        fz.plane_size = p->plane_size;
-        fz.nplanes = p->nplanes;
-        fz.slots = p->tw;
-        fz.bs = d.block_step; fz.tbx = p->ftbx; fz.tby = p->ftby; fz.ntx = p->fntx; fz.nty = p->fnty;
-        fz.rm = p->frm; fz.rw = p->frw; fz.rh = p->frh;
-        fz.accumulate = outliers ? 1 : 0;
-        fz.chroma = d.chroma;
-        for (int i = 0; i < 3; ++i) fz.sigma[i] = p->sigma[i];
-        BM3D_HIP(bm3d::launch_finalize(fz, nf * p->nplanes * p->tw, stream));
-
-        bm3d::DenArgs da{};
-        da.corner = p->corner;
-        da.corner_frame_stride = corner_eval;
-        da.res = res;
-        da.res_frame_stride = res_frame_stride;
-        da.res_plane_stride = res_plane_stride;
-        da.W = d.width; da.H = d.height; da.stride = d.stride;
-        da.plane_size = p->plane_size;
-        da.nplanes = p->nplanes;
-        da.slots = p->tw;
-        BM3D_HIP(bm3d::launch_den_box(da, nf * p->nplanes * p->tw, stream));
-    }
-    return BM3D_OK;
+    return run_slots(p, plan_slot_bufs(p), batch, d_res, res_frame_stride, res_plane_stride, src, ref, first_frame,
+                     count, stream);
 }
 
 long long bm3d_num_blocks(const bm3d_plan* p) { return p ? p->nblocks : 0; }
