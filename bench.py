@@ -19,20 +19,27 @@ the whole two-pass job over K steps, for the mixed-precision path
 rescoring of the best candidates). The plain FP32 path is timed the same way
 beside it (fp32_value, mixed_speedup = value / fp32_value).
 
-roofline: the block-matching kernel (dominant) of the reported path.
-Algorithmic FLOPs per launch = 192 x (in-plane candidates evaluated in that
-launch), the candidate count taken from the kernel's own counter; duration
-from HIP events bracketing each launch on its stream inside the timed region.
+roofline: the block-matching kernel (dominant) of the reported path, bound
+by VALU issue (FP16 dot2 pre-scores, FP32 rescoring; no MFMA). Algorithmic
+FLOPs per launch = 192 x (in-plane candidates evaluated in that launch), the
+candidate count taken from the kernel's own counter; duration from HIP
+events bracketing each launch on its stream inside the timed region; peak =
+MI355X's published vector peak, 157.3 TFLOP/s. roofline_filter: the same
+for the filtering kernel, with the reference's DCT flop counts per group.
 
 quality: PSNR of both paths against the clean clip; share of group lists
 the mixed path shares with the FP32 path; and, on a 192x160 crop of the
 first 5 frames, max-abs error and PSNR delta of both GPU paths against the
 FP64 CPU oracle running the same pipeline.
 
-cpu_baseline: the CPU oracle (a C restatement of the reference algorithm),
-frame-parallel over strips on host threads (one per core, at most 16), on a
-bounded sample: one basic + one final BM3D window (radius 1) for 4 frames,
-each split into 8 full-width strips of 135 rows (about 15 s of CPU work).
+cpu_baseline: the CPU oracle (a C restatement of the reference algorithm,
+scalar), frame-parallel on host threads (one per core, at most 16: a GPU
+box's CPU share), on a bounded sample: the basic and the final BM3D window
+(radius 1) of 8 whole 1080p frames as 16 independent jobs (about a minute of
+CPU work, a few seconds of wall time), the CPU model stated.
+
+batch_64_clips (main line; --no-batch skips it): the 64-clip mix below,
+per-clip FP32 plans against one autotuned mixed-precision batch graph.
 
 host workload (--workload host): the same clip in pageable host memory,
 through the plugin-level BM3Dv2 filters' host-frame route (whole-clip calls
@@ -82,38 +89,49 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(noisy, basic, threads):
-    """Oracle timing on host cores (see module docstring)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(noisy, basic, threads, frames=(4, 9, 14, 19, 24, 27, 10, 21)):
+    """Oracle timing on host cores (see module docstring): whole 1080p
+    windows, basic and final pass of each frame as independent jobs."""
     import numpy as np
     from oracle import oracle as O
 
     r = PARAMS["radius"]
     sig_b = [O.scale_sigma(PARAMS["sigma"], False)]
     sig_f = [O.scale_sigma(PARAMS["sigma"], True)]
-    frames = [5, 10, 15, 20]
-    rows = 135
     jobs = []
     for n in frames:
         win = np.ascontiguousarray(np.transpose(noisy[n - r:n + r + 1], (1, 0, 2, 3)))
         bwin = np.ascontiguousarray(np.transpose(basic[n - r:n + r + 1], (1, 0, 2, 3)))
-        for y0 in range(0, H, rows):
-            jobs.append((np.ascontiguousarray(win[:, :, y0:y0 + rows]),
-                         np.ascontiguousarray(bwin[:, :, y0:y0 + rows])))
+        jobs.append((win, None))
+        jobs.append((win, bwin))
 
     def work(job):
         s, b = job
-        O.bm3d_window(s, None, sig_b, PARAMS["block_step"], PARAMS["bm_range"], r,
-                      PARAMS["ps_num"], PARAMS["ps_range"], False, False)
-        O.bm3d_window(s, b, sig_f, PARAMS["block_step"], PARAMS["bm_range"], r,
-                      PARAMS["ps_num"], PARAMS["ps_range"], False, True)
+        if b is None:
+            O.bm3d_window(s, None, sig_b, PARAMS["block_step"], PARAMS["bm_range"], r,
+                          PARAMS["ps_num"], PARAMS["ps_range"], False, False)
+        else:
+            O.bm3d_window(s, b, sig_f, PARAMS["block_step"], PARAMS["bm_range"], r,
+                          PARAMS["ps_num"], PARAMS["ps_range"], False, True)
 
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         list(ex.map(work, jobs))
     dt = time.perf_counter() - t0
     return {"value": round(len(frames) / dt, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C, -O2) two-pass windows (basic+final, r=1) of {len(frames)} "
-                      f"1080p frames as {len(jobs)} strips of {rows} rows on {threads} threads; "
+            "sample": f"oracle (C, -O2, scalar) two-pass BM3D windows (basic and final, r=1) of "
+                      f"{len(frames)} whole 1080p frames, {len(jobs)} jobs on {threads} threads of "
+                      f"{cpu_model()} ({os.cpu_count()} logical CPUs visible; a GPU box's share is 16); "
                       f"{dt:.2f} s wall"}
 
 
@@ -173,6 +191,12 @@ def time_path(core, capi, src, precision, steps, warmup, autotune):
     avg_ms = match_ms / match_n
     flops_launch = flops_step / (match_n / steps)
     achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    # filter: the reference's DCT counts per group (hard 192 (44 + 51), Wiener
+    # 192 (2 44 + 51)) plus 6 per coefficient for shrinkage and aggregation
+    nblk = pipe.pb.num_blocks
+    f_flops_step = src.shape[0] * nblk * (192.0 * (44 + 51) + 192.0 * (2 * 44 + 51) + 2 * 512 * 6)
+    f_avg_ms = filt_ms / filt_n
+    f_flops_launch = f_flops_step / (filt_n / steps)
     log(f"{precision}: {steps} steps in {dt:.3f}s, match {match_ms / steps:.2f} ms/step, "
         f"filter {filt_ms / steps:.2f} ms/step")
     return {
@@ -180,6 +204,8 @@ def time_path(core, capi, src, precision, steps, warmup, autotune):
         "match_ms_step": match_ms / steps, "filter_ms_step": filt_ms / steps,
         "avg_launch_ms": avg_ms, "flops_launch": flops_launch, "achieved_tflops": achieved,
         "ncand": ncand, "pipe": pipe, "tuning": tuning,
+        "filter_avg_launch_ms": f_avg_ms, "filter_flops_launch": f_flops_launch,
+        "filter_achieved_tflops": f_flops_launch / (f_avg_ms * 1e-3) / 1e12,
     }
 
 
@@ -217,9 +243,11 @@ def batch_workload(n_clips=64, frames=8):
     return synth.batch_clips(n_clips, frames)
 
 
-def run_batch(args):
+def run_batch(args, main_line=False):
     """Aggregate Mpixel/s of the 64-clip mix: one batched graph (mixed
-    precision, and FP32) vs per-clip FP32 plan launches, two passes each."""
+    precision, and FP32) vs per-clip FP32 plan launches, two passes each.
+    main_line: only per-clip FP32 and the mixed graph, returned as a dict
+    for the main JSON line instead of printed."""
     import torch
     from bm3d_amd import core, synth
 
@@ -258,7 +286,7 @@ def run_batch(args):
     ref_basic = [b.clone() for b in basics]
     del plans
     res = {"per_clip_fp32_mpixel_s": round(px / t_clip / 1e6, 1)}
-    for prec in ("fp32", "mixed"):
+    for prec in (("mixed",) if main_line else ("fp32", "mixed")):
         jobs = [dict(synth.BATCH_PARAMS, src=s, out=o, basic=b, passes=2, radius=c["radius"], sigma=[c["sigma"]],
                      precision=prec) for c, s, o, b in zip(clips, srcs, outs, basics)]
         batch = core.Batch(jobs)
@@ -279,6 +307,13 @@ def run_batch(args):
             res["batch_fp32_basic_max_abs_vs_per_clip"] = max(float((b - r).abs().max())
                                                               for b, r in zip(basics, ref_basic))
         del batch
+    if main_line:
+        return {"workload": "64 GRAYS clips x 8 frames, 360p..2160p, radius U{0,1,2}, sigma U[5,25], "
+                            "two passes, one captured graph (autotuned)", "pixels_per_run": px,
+                "mixed_mpixel_s": res["batch_mixed_mpixel_s"], "mixed_ms": res["batch_mixed_ms"],
+                "per_clip_fp32_mpixel_s": res["per_clip_fp32_mpixel_s"],
+                "speedup_vs_per_clip_fp32": round(res["batch_mixed_mpixel_s"] / res["per_clip_fp32_mpixel_s"], 3),
+                "graph": res["batch_mixed_graph"]}
     out = {
         "metric": "aggregate Mpixel/s, 64-clip batch of two-pass BM3Dv2 (360p..2160p, radius 0..2)",
         "value": res["batch_mixed_mpixel_s"],
@@ -450,6 +485,8 @@ def main():
                     help="vbm3d: the 1080p two-pass V-BM3D clip (default); batch: the 64-clip mix; "
                          "host: the vbm3d clip from host memory through the plugin-level filters")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-clip batch figures in the main line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -482,6 +519,7 @@ def main():
     res = {p: time_path(core, _capi, src, p, args.steps, args.warmup, not args.no_autotune) for p in paths}
     main_r = res[args.precision]
 
+    basic_host = main_r["pipe"].basic.cpu().numpy()
     quality = {}
     for p, r in res.items():
         quality[f"psnr_final_{p}_db"] = round(synth.psnr(r["pipe"].final.cpu().numpy(), clean), 4)
@@ -516,17 +554,33 @@ def main():
                    "launch_shapes": res[args.precision]["tuning"]},
         "mpixel_per_s": round(main_r["fps"] * W * H / 1e6, 2),
         "roofline": {
-            "bound": "mfma",
-            "compute_pipe": "VALU (FP32 exact / packed FP16 preselection); packed FP32 disabled",
-            "kernel": "bm3d_match",
+            "bound": "valu",
+            "compute_pipe": "VALU: FP16 v_dot2 pre-scores (mixed) / FP32 FMA (fp32); peak = MI355X "
+                            "published vector peak (FP32 with packed math, = FP16 dot2 rate); no MFMA",
+            "kernel": "bm3d_match_dx" if args.precision == "mixed" else "bm3d_match",
             "achieved": round(main_r["achieved_tflops"], 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(main_r["achieved_tflops"] / PEAK_FP32_TFLOPS, 4),
-            "traffic": profiled_traffic("bm3d_match", args.precision),
+            "traffic": profiled_traffic("bm3d_match_dx" if args.precision == "mixed" else "bm3d_match",
+                                        args.precision),
             "flops_per_launch": main_r["flops_launch"],
             "avg_launch_ms": round(main_r["avg_launch_ms"], 4),
             "candidates_per_step": main_r["ncand"],
+        },
+        "roofline_filter": {
+            "bound": "valu",
+            "compute_pipe": "VALU + LDS (FP16 MFMA for the 3-D transforms in the mixed kernel); achieved "
+                            "counts the reference's DCT flops, not the MFMA products",
+            "kernel": "bm3d_filter_mx" if args.precision == "mixed" else "bm3d_filter",
+            "achieved": round(main_r["filter_achieved_tflops"], 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(main_r["filter_achieved_tflops"] / PEAK_FP32_TFLOPS, 4),
+            "traffic": profiled_traffic("bm3d_filter_mx" if args.precision == "mixed" else "bm3d_filter",
+                                        args.precision),
+            "flops_per_launch": main_r["filter_flops_launch"],
+            "avg_launch_ms": round(main_r["filter_avg_launch_ms"], 4),
         },
         "kernel_ms_per_step": {p: {"bm3d_match": round(r["match_ms_step"], 3),
                                    "bm3d_filter": round(r["filter_ms_step"], 3)}
@@ -537,10 +591,13 @@ def main():
         out["fp32_value"] = round(res["fp32"]["fps"], 3)
         out["mixed_value"] = round(res["mixed"]["fps"], 3)
         out["mixed_speedup"] = round(res["mixed"]["fps"] / res["fp32"]["fps"], 4)
+    if not args.no_batch:
+        del res
+        torch.cuda.empty_cache()
+        out["batch_64_clips"] = run_batch(args, main_line=True)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        bas = res[args.precision]["pipe"].basic.cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(noisy, bas, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(noisy, basic_host, args.cpu_threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
