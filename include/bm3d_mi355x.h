@@ -111,7 +111,11 @@ typedef struct bm3d_clip {
 } bm3d_clip;
 
 /* Creates a plan on the current HIP device. Validates the parameters with
- * the reference's messages (source/source.cpp:476-611). */
+ * the reference's messages (source/source.cpp:476-611). Any search reach the
+ * reference accepts is planned: when bm_range + radius * ps_range outgrows
+ * the LDS tiles, the filter adds the members beyond its accumulation margin
+ * with global atomics (bm3d_execute_vagg and batches then refuse the plan),
+ * and the matcher reads the planes from global memory. */
 int bm3d_plan_create(const bm3d_desc *desc, bm3d_plan **plan);
 void bm3d_plan_destroy(bm3d_plan *plan);
 

@@ -437,8 +437,9 @@ int apply_to_jobs(bm3d_batch* b, const bm3d_tuning& t) {
                 if (int rc = bm3d::plan_apply_tuning(p, t)) return rc;
     for (Job& j : b->jobs)
         for (bm3d_plan* p : j.plan)
-            if (p && p->frm < p->margin)
-                return fail(BM3D_EUNSUPPORTED, "BM3D: search reach too large for a batch job's filter tiles");
+            if (p && (p->frm < p->margin || p->lds_bytes == 0))
+                return fail(BM3D_EUNSUPPORTED, "BM3D: search reach too large for a batch job's LDS tiles "
+                                               "(use per-clip plans)");
     b->tune = t;
     return BM3D_OK;
 }

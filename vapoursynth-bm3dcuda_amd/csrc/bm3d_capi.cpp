@@ -268,7 +268,7 @@ bool match_geometry(bm3d_plan* p, int rows, int width) {
 void dx_geometry(bm3d_plan* p) {
     const bm3d_desc& d = p->d;
     p->dx_ok = false;
-    if (d.precision != BM3D_PRECISION_MIXED || d.bm_error != BM3D_BM_SSD) return;
+    if (d.precision != BM3D_PRECISION_MIXED || d.bm_error != BM3D_BM_SSD || p->lds_bytes == 0) return;
     const int nWs = 2 * d.bm_range + 1, nWt = 2 * d.ps_range + 1;
     if (nWs * nWs > 4096 || (d.radius > 0 && d.ps_num * nWt * nWt > 4096)) return;
     p->tile_hp2 = ((p->tile_h + 1) / 2 + 1) | 1;
@@ -297,12 +297,18 @@ bool filter_geometry(bm3d_plan* p, int tbx, int tby, int rm) {
 // Applies a tuning's tiles: the requested ones at the full search reach
 // when they fit, else the largest default tile that fits, else a 4 x 4 tile
 // with a reduced margin (members beyond it go to global atomics, and
-// bm3d_execute_vagg is unavailable).
+// bm3d_execute_vagg is unavailable). A matching tile whose union window
+// does not fit even 4 rows of blocks makes the matcher read the planes from
+// global memory (no LDS tile; same results, slower).
 int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
     if (p->pinned)
         return fail(BM3D_EINVAL, "BM3D: the plan is held by a captured graph; its launch shapes cannot change");
-    if (!match_geometry(p, t.match_rows, t.match_width) && !match_geometry(p, 4, 16))
-        return fail(BM3D_EUNSUPPORTED, "BM3D: search window (bm_range + radius * ps_range) too large for LDS");
+    if (!match_geometry(p, t.match_rows, t.match_width) && !match_geometry(p, 4, 16)) {
+        p->tbw = t.match_width;
+        p->tby = t.match_rows;
+        p->tile_w = p->tile_h = p->tile_wq = p->tile_hp = 0;
+        p->lds_bytes = 0;
+    }
     dx_geometry(p);
     if (!p->dx_ok) p->use_dx = false;
     bool ok = filter_geometry(p, t.filter_tile_x, t.filter_tile_y, p->margin);

@@ -33,7 +33,7 @@ struct bm3d_plan {
     bm3d_tuning tune{};
     // matching work-group tile: tby rows of tbw reference blocks plus the margin
     int tbw = 16, tby = 16, tile_w = 0, tile_h = 0, tile_wq = 0, tile_hp = 0;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;  // 0: the union window exceeds the LDS; the matcher reads the planes
     // filter work-group tiling and LDS accumulation region (margin frm)
     int ftbx = 8, ftby = 8, fntx = 0, fnty = 0, frm = 0, frw = 0, frh = 0;
     size_t filter_lds_bytes = 0;

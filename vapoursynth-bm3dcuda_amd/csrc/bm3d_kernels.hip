@@ -154,8 +154,10 @@ __device__ __forceinline__ void xform8_strided(float (&v)[8][8], int c, int kind
 // insert (source/kernel.cu:616-647). Writes 8 packed positions per block;
 // with kCount, also adds the number of in-plane candidates to *a.ncand.
 // kCost selects the block distance (MatchArgs::cost, the bm_error_s option
-// of rtc_source/kernel.hpp:372-504).
-template <int kCost, bool kCount>
+// of rtc_source/kernel.hpp:372-504). kPlane: the candidates are read from
+// the planes in global memory instead of an LDS tile (search reaches whose
+// union window does not fit the LDS); same order, same distances.
+template <int kCost, bool kCount, bool kPlane>
 __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     extern __shared__ float lds[];
     const int wshift = a.tbw == 32 ? 5 : 4;
@@ -188,11 +190,16 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
         return a.clip + (size_t)fi * a.frame_stride;
     };
 
-    stage_tile(lds, g, slot_plane(r), a.W, a.H, a.stride);
-    __syncthreads();
-
     float rb[8][8];
-    {
+    if constexpr (kPlane) {
+        const float* blk = slot_plane(r) + (size_t)y * a.stride + x;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) rb[j][c] = blk[(size_t)j * a.stride + c];
+    } else {
+        stage_tile(lds, g, slot_plane(r), a.W, a.H, a.stride);
+        __syncthreads();
         const int tc = x - g.tx0;
         const int rt = y - g.ty0;
 #pragma unroll
@@ -209,7 +216,10 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     unsigned cnt = 0;
     Best8 best;
     best.init(kFltMax, pack_pos(x, y, r));
-    scan_window<kCost, kCount>(lds, g, rb, st, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+    if constexpr (kPlane)
+        scan_window_plane<kCost, kCount>(slot_plane(r), a.stride, rb, st, best, x, y, a.bm_range, r, a.W, a.H, cnt);
+    else
+        scan_window<kCost, kCount>(lds, g, rb, st, best, x, y, a.bm_range, r, a.W, a.H, cnt);
 
     if (r > 0) {
         uint32_t centre[8];
@@ -221,17 +231,23 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
             for (int i = 0; i < 8; ++i) last[i] = centre[i];
             for (int t = 1; t <= r; ++t) {
                 const int z = r + dir * t;
-                __syncthreads();
-                stage_tile(lds, g, slot_plane(z), a.W, a.H, a.stride);
-                __syncthreads();
+                if constexpr (!kPlane) {
+                    __syncthreads();
+                    stage_tile(lds, g, slot_plane(z), a.W, a.H, a.stride);
+                    __syncthreads();
+                }
                 Best8 fr;
                 fr.init(kFltMax, pack_pos(0, 0, z));
                 for (int i = 0; i < a.ps_num; ++i) {
                     uint32_t p = last[0];
 #pragma unroll
                     for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                    scan_window<kCost, kCount>(lds, g, rb, st, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H,
-                                               cnt);
+                    if constexpr (kPlane)
+                        scan_window_plane<kCost, kCount>(slot_plane(z), a.stride, rb, st, fr, pos_x(p), pos_y(p),
+                                                         a.ps_range, z, a.W, a.H, cnt);
+                    else
+                        scan_window<kCost, kCount>(lds, g, rb, st, fr, pos_x(p), pos_y(p), a.ps_range, z, a.W, a.H,
+                                                   cnt);
                 }
                 for (int i = 0; i < a.ps_num; ++i) {
                     const float e = fr.err_at(i);
@@ -265,9 +281,9 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
     }
 }
 
-template <int kCost, bool kCount>
+template <int kCost, bool kCount, bool kPlane>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
-    match_body<kCost, kCount>(a, launch_wg());
+    match_body<kCost, kCount, kPlane>(a, launch_wg());
 }
 
 template <int kCost>
@@ -275,7 +291,7 @@ __global__ __launch_bounds__(256) void bm3d_match_multi(const MatchArgs* __restr
                                                         const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
-    match_body<kCost, false>(a, job_wg(grids[j], blockIdx.x));
+    match_body<kCost, false, false>(a, job_wg(grids[j], blockIdx.x));
 }
 
 // ---------------------------------------------------------------- filtering
@@ -882,10 +898,16 @@ __global__ __launch_bounds__(256) void bm3d_vagg_targets_multi(const VTargetArgs
 
 template <int kCost>
 void launch_match_cost(const MatchArgs& a, dim3 grid, dim3 block, size_t lds_bytes, hipStream_t s) {
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match<kCost, true>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match<kCost, false>), grid, block, lds_bytes, s, a);
+    if (lds_bytes == 0) {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match<kCost, true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((bm3d_match<kCost, false, true>), grid, block, 0, s, a);
+    } else {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match<kCost, true, false>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_match<kCost, false, false>), grid, block, lds_bytes, s, a);
+    }
 }
 
+// lds_bytes 0: the plane-reading variant.
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
@@ -916,8 +938,8 @@ hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int n
 
 template <int kCost>
 hipError_t set_match_cost_lds_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<kCost, false>),
-                         reinterpret_cast<const void*>(&bm3d_match<kCost, true>),
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match<kCost, false, false>),
+                         reinterpret_cast<const void*>(&bm3d_match<kCost, true, false>),
                          reinterpret_cast<const void*>(&bm3d_match_multi<kCost>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

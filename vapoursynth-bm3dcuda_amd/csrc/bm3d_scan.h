@@ -223,5 +223,54 @@ __device__ __forceinline__ void scan_window(const float* __restrict__ lds, const
     }
 }
 
+// scan_window reading one plane in global memory instead of an LDS tile,
+// for search reaches whose union window does not fit the LDS: the same
+// candidates in the same order and the same distances, each candidate
+// column loaded as 8 strided words. Pixels outside the plane read the
+// clamped border, as the LDS tile holds; they are only ever read for
+// candidates that are masked off.
+template <int kCost, bool kCount>
+__device__ __forceinline__ void scan_window_plane(const float* __restrict__ img, int stride,
+                                                  const float (&rb)[8][8], const BlockStat& st, Best8& best,
+                                                  int cx, int cy, int range, int z, int W, int H, unsigned& cnt) {
+    const int n = 2 * range + 1;
+    float ring[8][8];
+    for (int dyi = 0; dyi < n; ++dyi) {
+        const int row = cy - range + dyi;
+        const bool row_ok = row >= 0 && row <= H - 8;
+        int ro[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ro[j] = clampi(row + j, 0, H - 1) * stride;
+        auto load = [&](float(&dst)[8], int col) {
+            const int c = clampi(col, 0, W - 1);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dst[j] = img[ro[j] + c];
+        };
+        const int col0 = cx - range;
+#pragma unroll
+        for (int s = 0; s < 7; ++s) load(ring[s], col0 + s);
+        for (int base = 0; base < n; base += 8) {
+#define BM3D_PLANE_STEP(U)                                                       \
+    if (base + U < n) {                                                          \
+        load(ring[(U + 7) & 7], col0 + base + U + 7);                            \
+        const float e = ring_cost<kCost, U>(rb, ring, st);                       \
+        const int col = col0 + base + U;                                         \
+        const bool ok = row_ok && col >= 0 && col <= W - 8;                      \
+        if constexpr (kCount) cnt += ok ? 1u : 0u;                               \
+        if (ok && e < best.e[7]) best.insert(e, pack_pos(col, row, z));          \
+    }
+            BM3D_PLANE_STEP(0)
+            BM3D_PLANE_STEP(1)
+            BM3D_PLANE_STEP(2)
+            BM3D_PLANE_STEP(3)
+            BM3D_PLANE_STEP(4)
+            BM3D_PLANE_STEP(5)
+            BM3D_PLANE_STEP(6)
+            BM3D_PLANE_STEP(7)
+#undef BM3D_PLANE_STEP
+        }
+    }
+}
+
 }  // namespace dev
 }  // namespace bm3d
