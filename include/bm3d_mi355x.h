@@ -434,6 +434,22 @@ int bm3d_filter_host_range(const bm3d_filter *filter, int first, int count, int 
 int bm3d_filter_get_frames_host(bm3d_filter *filter, const bm3d_host_frame *clip, const bm3d_host_frame *ref,
                                 int lo, bm3d_host_frame *out, int first, int count);
 
+/* ---- failure injection (tests) -------------------------------------------
+ * No counterpart in the reference, which reports runtime failures through
+ * setError / setFilterError (source/source.cpp:54-59, 305-314, 225-231);
+ * this hook lets a test force those failures. Arms a one-shot countdown for
+ * one kind of runtime event: device allocations (hipMalloc), pinned host
+ * allocations (the host-frame route's staging buffers) or kernel launches.
+ * The event that finds the countdown at zero (after = 0: the next one) fails
+ * with hipErrorOutOfMemory / hipErrorLaunchFailure without reaching the
+ * runtime, and the call that made it returns BM3D_EHIP with a message naming
+ * it; later calls work normally. after < 0 disarms the site. Process-wide:
+ * events on every thread count. */
+#define BM3D_INJECT_DEVICE_ALLOC 1
+#define BM3D_INJECT_HOST_ALLOC 2
+#define BM3D_INJECT_LAUNCH 3
+int bm3d_debug_inject_failure(int site, long long after);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *bm3d_last_error(void);
 

@@ -190,6 +190,8 @@ def lib():
     PH = ctypes.POINTER(HostFrame)
     L.bm3d_filter_get_frames_host.argtypes = [P, PH, PH, c_int, PH, c_int, c_int]
     L.bm3d_filter_get_frames_host.restype = c_int
+    L.bm3d_debug_inject_failure.argtypes = [c_int, c_ll]
+    L.bm3d_debug_inject_failure.restype = c_int
     L.bm3d_last_error.argtypes = []
     L.bm3d_last_error.restype = ctypes.c_char_p
     L.bm3d_version.argtypes = []
@@ -217,8 +219,12 @@ EXPORTS = (
     "bm3d_batch_autotune", "bm3d_batch_info", "bm3d_batch_destroy",
     "bm3d_filter_create", "bm3d_vaggregate_create", "bm3dv2_filter_create", "bm3d_filter_info",
     "bm3d_filter_get_frames", "bm3d_filter_destroy", "bm3d_filter_host_range", "bm3d_filter_get_frames_host",
-    "bm3d_execute_noalloc",
+    "bm3d_execute_noalloc", "bm3d_debug_inject_failure",
 )
+
+INJECT_DEVICE_ALLOC = 1
+INJECT_HOST_ALLOC = 2
+INJECT_LAUNCH = 3
 
 KERNEL_MATCH = 0
 KERNEL_FILTER = 1

@@ -72,7 +72,38 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_plugins()
     return LIB
+
+
+VSPLUGIN = os.path.join(HERE, "vsplugin")
+CXX = os.environ.get("CXX", "g++")
+# VapourSynth plugin entries over the library: (output, extra flags). The
+# first serves the bm3dcuda namespace, the second bm3dcuda_rtc (the NVRTC
+# variant's string options).
+PLUGINS = [
+    ("libbm3dcuda_mi355x.so", []),
+    ("libbm3dcuda_rtc_mi355x.so", ["-DBM3D_VS_RTC"]),
+]
+
+
+def plugin_cmd(out, extra, sanitize=False):
+    """g++ line of one plugin entry (host code only: it calls the C ABI)."""
+    lib_dir = os.path.dirname(out)
+    san = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-g"] if sanitize else []
+    return [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", *san, *extra,
+            f"-I{os.path.join(ROOT, 'include')}", f"-I{VSPLUGIN}", os.path.join(VSPLUGIN, "bm3d_vsplugin.cpp"),
+            f"-L{lib_dir}", "-lbm3d_mi355x", "-Wl,-rpath,$ORIGIN", "-o", out]
+
+
+def build_plugins():
+    for name, extra in PLUGINS:
+        out = os.path.join(OUT_DIR, name)
+        cmd = plugin_cmd(out + ".tmp", extra)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"plugin build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -21,6 +21,7 @@
 #include <tuple>
 #include <vector>
 
+#include "bm3d_inject.h"
 #include "bm3d_internal.h"
 
 namespace {
@@ -396,11 +397,11 @@ int rebuild(bm3d_batch* b) {
     layout(b);
     if (b->arena) (void)hipFree(b->arena);
     b->arena = nullptr;
-    BATCH_HIP(hipMalloc(&b->arena, std::max<size_t>(b->arena_bytes, 256)));
+    BATCH_HIP(bm3d::malloc_device(&b->arena, std::max<size_t>(b->arena_bytes, 256)));
     std::vector<char> tab;
     if (int rc = build_launches(b, tab)) return rc;
     b->tables_bytes = std::max<size_t>(tab.size(), 256);
-    BATCH_HIP(hipMalloc(&b->tables, b->tables_bytes));
+    BATCH_HIP(bm3d::malloc_device(&b->tables, b->tables_bytes));
     if (!tab.empty()) BATCH_HIP(hipMemcpy(b->tables, tab.data(), tab.size(), hipMemcpyHostToDevice));
 
     hipStream_t cs = nullptr;

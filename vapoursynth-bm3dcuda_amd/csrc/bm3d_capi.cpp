@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <limits>
@@ -19,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "bm3d_inject.h"
 #include "bm3d_internal.h"
 #include "bm3d_kernels.h"
 #include "bm3d_mi355x.h"
@@ -349,7 +351,7 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch, h
         if (p->groups) (void)hipFree(p->groups);
         p->groups = nullptr;
         p->groups_cap = 0;
-        BM3D_HIP(hipMalloc(&p->groups, groups * sizeof(uint32_t)));
+        BM3D_HIP(bm3d::malloc_device(&p->groups, groups * sizeof(uint32_t)));
         p->groups_cap = groups;
     }
     if (corner > p->corner_cap) {
@@ -358,8 +360,8 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch, h
         p->corner = nullptr;
         p->wrap = nullptr;
         p->corner_cap = 0;
-        BM3D_HIP(hipMalloc(&p->corner, corner * sizeof(float)));
-        BM3D_HIP(hipMalloc(&p->wrap, corner * sizeof(int)));
+        BM3D_HIP(bm3d::malloc_device(&p->corner, corner * sizeof(float)));
+        BM3D_HIP(bm3d::malloc_device(&p->wrap, corner * sizeof(int)));
         // counts start at zero: readers use them whenever the flag is set
         BM3D_HIP(hipMemset(p->wrap, 0, corner * sizeof(int)));
         p->corner_cap = corner;
@@ -368,14 +370,14 @@ int ensure_buffers(bm3d_plan* p, size_t groups, size_t corner, size_t scratch, h
         if (p->scratch) (void)hipFree(p->scratch);
         p->scratch = nullptr;
         p->scratch_cap = 0;
-        BM3D_HIP(hipMalloc(&p->scratch, scratch * sizeof(int)));
+        BM3D_HIP(bm3d::malloc_device(&p->scratch, scratch * sizeof(int)));
         p->scratch_cap = scratch;
     }
     if (part > p->part_cap) {
         if (p->part) (void)hipFree(p->part);
         p->part = nullptr;
         p->part_cap = 0;
-        BM3D_HIP(hipMalloc(&p->part, part * sizeof(long long)));
+        BM3D_HIP(bm3d::malloc_device(&p->part, part * sizeof(long long)));
         p->part_cap = part;
     }
     return BM3D_OK;
@@ -669,7 +671,32 @@ VTargetArgs plan_vtarget_args(const bm3d_plan* p) { return vtarget_args(p, 1); }
 
 }  // namespace bm3d
 
+namespace bm3d {
+
+namespace {
+std::atomic<long long> g_inject[kInjectSites] = {-1, -1, -1, -1};
+}
+
+bool inject_now(int site) {
+    std::atomic<long long>& c = g_inject[site];
+    long long v = c.load(std::memory_order_relaxed);
+    while (v >= 0) {
+        if (c.compare_exchange_weak(v, v - 1, std::memory_order_relaxed)) return v == 0;
+    }
+    return false;
+}
+
+}  // namespace bm3d
+
 extern "C" {
+
+int bm3d_debug_inject_failure(int site, long long after) {
+    g_last_error.clear();
+    if (site < bm3d::kInjectDeviceAlloc || site >= bm3d::kInjectSites)
+        return fail(BM3D_EINVAL, "BM3D: unknown failure injection site");
+    bm3d::g_inject[site].store(after < 0 ? -1 : after, std::memory_order_relaxed);
+    return BM3D_OK;
+}
 
 const char* bm3d_last_error(void) { return g_last_error.c_str(); }
 
@@ -744,7 +771,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     }
     p->use_dx = p->dx_ok;
     p->vbound = bm3d::contribution_bound(d.transform_2d, d.transform_1d, p->d.final_ != 0);
-    e = hipMalloc(&p->range, 256);  // the range, then the wrap flag
+    e = bm3d::malloc_device(&p->range, 256);  // the range, then the wrap flag
     if (e == hipSuccess) e = hipMemset(p->range, 0, 256);
     if (e != hipSuccess) {
         bm3d_plan_destroy(p);
@@ -756,7 +783,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);
         bm3d::build_filter_mx_ops(ops.data(), d.transform_2d, d.transform_1d);
         const size_t bytes = ops.size() * sizeof(uint32_t);
-        e = hipMalloc(&p->mx_ops, bytes);
+        e = bm3d::malloc_device(&p->mx_ops, bytes);
         if (e == hipSuccess) e = hipMemcpy(p->mx_ops, ops.data(), bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             bm3d_plan_destroy(p);
@@ -928,7 +955,7 @@ int bm3d_plan_autotune(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref,
     const bool fused = d.radius > 0 && p->frm >= p->margin;
     const size_t out_floats = fused ? (size_t)count * p->nplanes * d.height * d.width : (size_t)count * p->stack_size;
     float* out = nullptr;
-    BM3D_HIP(hipMalloc(&out, out_floats * sizeof(float)));
+    BM3D_HIP(bm3d::malloc_device(&out, out_floats * sizeof(float)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
@@ -1013,7 +1040,7 @@ int bm3d_count_candidates(bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* r
         return fail(BM3D_EINVAL, "BM3D: frame range out of bounds");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     unsigned long long* d_cnt = nullptr;
-    BM3D_HIP(hipMalloc(&d_cnt, sizeof(unsigned long long)));
+    BM3D_HIP(bm3d::malloc_device(&d_cnt, sizeof(unsigned long long)));
     hipError_t e = bm3d::launch_zero(reinterpret_cast<float*>(d_cnt), 2, 0, 1, st);  // the counter's 64 bits
     if (e == hipSuccess && count > 0) e = launch_matching(p, nullptr, d_cnt, src, ref, first_frame, count, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ncand, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);

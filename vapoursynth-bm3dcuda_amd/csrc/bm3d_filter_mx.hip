@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <stdint.h>
 
+#include "bm3d_inject.h"
 #include "bm3d_device.h"
 #include "bm3d_xform.h"
 
@@ -488,6 +489,7 @@ void launch_filter_mx_kind(const FilterArgs& a, bool final_, bool chroma, dim3 g
 // of them returns at once.
 hipError_t launch_filter_mx(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                             hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(a.ntx, a.nty, grid_z), block(64 * waves);
     launch_filter_mx_kind<false>(a, final_, chroma, grid, block, lds_bytes, s);
     if (a.extractor != 0.0f) launch_filter_mx_kind<true>(a, final_, chroma, grid, block, lds_bytes, s);
@@ -515,6 +517,7 @@ void launch_filter_mx_multi_kind(const FilterArgs* jobs, const JobGrid* grids, i
 
 hipError_t launch_filter_mx_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                   bool chroma, int waves, size_t lds_bytes, hipStream_t s, bool any_extractor) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(wgs), block(64 * waves);
     launch_filter_mx_multi_kind<false>(jobs, grids, njobs, grid, block, final_, chroma, lds_bytes, s);
     if (any_extractor) launch_filter_mx_multi_kind<true>(jobs, grids, njobs, grid, block, final_, chroma, lds_bytes, s);

@@ -13,6 +13,7 @@
 #include <memory>
 #include <string>
 
+#include "bm3d_inject.h"
 #include "bm3d_internal.h"
 #include "bm3d_mi355x.h"
 
@@ -51,7 +52,7 @@ int ensure_scratch(bm3d_filter* f, size_t floats) {
     if (f->scratch) (void)hipFree(f->scratch);
     f->scratch = nullptr;
     f->scratch_cap = 0;
-    BM3D_HIP_F(hipMalloc(&f->scratch, floats * sizeof(float)));
+    BM3D_HIP_F(bm3d::malloc_device(&f->scratch, floats * sizeof(float)));
     f->scratch_cap = floats;
     return BM3D_OK;
 }

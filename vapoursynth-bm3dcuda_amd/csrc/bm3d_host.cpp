@@ -32,6 +32,7 @@
 #include <thread>
 #include <vector>
 
+#include "bm3d_inject.h"
 #include "bm3d_internal.h"
 
 using bm3d::fail;
@@ -219,7 +220,7 @@ int grow_device(float** p, size_t* cap, size_t floats) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     *cap = 0;
-    BM3D_HOST_HIP(hipMalloc(p, floats * sizeof(float)));
+    BM3D_HOST_HIP(bm3d::malloc_device(p, floats * sizeof(float)));
     *cap = floats;
     return BM3D_OK;
 }
@@ -231,7 +232,7 @@ int grow_pinned(float* (&p)[2], size_t* cap, size_t floats) {
         q = nullptr;
     }
     *cap = 0;
-    for (auto& q : p) BM3D_HOST_HIP(hipHostMalloc(reinterpret_cast<void**>(&q), floats * sizeof(float), 0));
+    for (auto& q : p) BM3D_HOST_HIP(bm3d::malloc_pinned(&q, floats * sizeof(float)));
     *cap = floats;
     return BM3D_OK;
 }

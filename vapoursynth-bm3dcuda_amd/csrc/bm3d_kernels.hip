@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bm3d_inject.h"
 #include "bm3d_scan.h"
 #include "bm3d_xform.h"
 
@@ -909,6 +910,7 @@ void launch_match_cost(const MatchArgs& a, dim3 grid, dim3 block, size_t lds_byt
 
 // lds_bytes 0: the plane-reading variant.
 hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
     switch (a.cost) {
@@ -924,6 +926,7 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, int cost, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(wgs), block(threads);
     switch (cost) {
         case 0: hipLaunchKernelGGL((bm3d_match_multi<0>), grid, block, lds_bytes, s, jobs, grids, njobs); break;
@@ -971,6 +974,7 @@ void launch_filter_kind(const FilterArgs& a, bool final_, bool chroma, dim3 grid
 
 hipError_t launch_filter(const FilterArgs& a, bool final_, bool chroma, int grid_z, int waves, size_t lds_bytes,
                          hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(a.ntx, a.nty, grid_z);
     dim3 block(64 * waves);
     if (a.t2d < 0 || a.t2d >= kNumTransforms || a.t1d < 0 || a.t1d >= kNumTransforms) return hipErrorInvalidValue;
@@ -1000,6 +1004,7 @@ void launch_filter_multi_kind(const FilterArgs* jobs, const JobGrid* grids, int 
 
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(wgs), block(64 * waves);
     if (generic) launch_filter_multi_kind<true>(jobs, grids, njobs, final_, chroma, grid, block, lds_bytes, s);
     else launch_filter_multi_kind<false>(jobs, grids, njobs, final_, chroma, grid, block, lds_bytes, s);
@@ -1030,12 +1035,14 @@ hipError_t set_filter_lds_limit(size_t bytes) {
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, int count, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.W + 255) / 256, a.H, count);
     hipLaunchKernelGGL(bm3d_finalize, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s, const int* flag) {
+    BM3D_INJECT_LAUNCH();
     if (n == 0 || count == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -1045,6 +1052,7 @@ hipError_t launch_zero(float* p, size_t n, size_t step, int count, hipStream_t s
 
 hipError_t launch_absmax(const float* clip, long long frame_stride, long long plane_stride, int nplanes, int W,
                          int H, int stride, int frames, float* out, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid(H < 64 ? H : 64, frames * nplanes);
     hipLaunchKernelGGL(bm3d_absmax, grid, dim3(256), 0, s, clip, frame_stride, plane_stride, nplanes, W, H, stride,
                        out);
@@ -1052,12 +1060,14 @@ hipError_t launch_absmax(const float* clip, long long frame_stride, long long pl
 }
 
 hipError_t launch_den_box(const DenArgs& a, int count, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.W + 63) / 64, (a.H + 7) / 8, count);
     hipLaunchKernelGGL(bm3d_den_box, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.W + 255) / 256, a.H, count);
     hipLaunchKernelGGL(bm3d_divide, grid, dim3(256), 0, s, a);
     return hipGetLastError();
@@ -1065,22 +1075,26 @@ hipError_t launch_divide(const DivideArgs& a, int count, hipStream_t s) {
 
 hipError_t launch_vagg_targets_multi(const VTargetArgs* jobs, const JobGrid* grids, int njobs, int wgs,
                                      hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     hipLaunchKernelGGL(bm3d_vagg_targets_multi, dim3(wgs), dim3(256), 0, s, jobs, grids, njobs);
     return hipGetLastError();
 }
 
 hipError_t launch_absmax_multi(const AbsmaxArgs* jobs, const JobGrid* grids, int njobs, int wgs, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     hipLaunchKernelGGL(bm3d_absmax_multi, dim3(wgs), dim3(256), 0, s, jobs, grids, njobs);
     return hipGetLastError();
 }
 
 hipError_t launch_vagg_targets(const VTargetArgs& a, int count, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.W + 63) / 64, (a.H + kVaggRows - 1) / kVaggRows, count * a.nplanes);
     hipLaunchKernelGGL(bm3d_vagg_targets, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_vaggregate(const VAggArgs& a, int count, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.W + 255) / 256, a.H, count);
     hipLaunchKernelGGL(bm3d_vaggregate, grid, dim3(256), 0, s, a);
     return hipGetLastError();

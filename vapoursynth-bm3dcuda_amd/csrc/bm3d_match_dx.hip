@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bm3d_inject.h"
 #include "bm3d_scan.h"
 
 namespace bm3d {
@@ -633,6 +634,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
 namespace {
 template <int kKt, bool kDiv8>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
     if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kDiv8>), grid, block, lds_bytes, s, a);
@@ -642,6 +644,7 @@ hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, 
 template <int kKt, bool kDiv8>
 hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                            size_t lds_bytes, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     hipLaunchKernelGGL((bm3d_match_dx_multi<kKt, kDiv8>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids,
                        njobs);
     return hipGetLastError();
@@ -659,6 +662,7 @@ hipError_t set_dx_limit(size_t bytes) {
 }
 template <bool kDiv8>
 hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     switch (a.kt) {
         case 4: return launch_dx<4, kDiv8>(a, tby, lds_bytes, frames, s);
         case 8: return launch_dx<8, kDiv8>(a, tby, lds_bytes, frames, s);
@@ -668,6 +672,7 @@ hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int fra
 template <bool kDiv8>
 hipError_t launch_dx_multi_keys(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                 size_t lds_bytes, int kt, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     switch (kt) {
         case 4: return launch_dx_multi<4, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
         case 8: return launch_dx_multi<8, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
@@ -681,12 +686,14 @@ int dx_temporal_keys(int ps_num) { return ps_num <= 4 ? 8 : 16; }
 bool dx_block_step_divides_8(int block_step) { return 8 % block_step == 0; }
 
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     return dx_block_step_divides_8(a.bs) ? launch_dx_keys<true>(a, tby, lds_bytes, frames, s)
                                          : launch_dx_keys<false>(a, tby, lds_bytes, frames, s);
 }
 
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                  size_t lds_bytes, int kt, bool div8, hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
     return div8 ? launch_dx_multi_keys<true>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s)
                 : launch_dx_multi_keys<false>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
 }
