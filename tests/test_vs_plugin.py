@@ -43,8 +43,18 @@ VAGG_ARGS = "clip:clip;src:clip;planes:int[];"
 API_VERSION = (3 << 16) | 6
 
 
-@pytest.fixture(scope="module")
-def host(tmp_path_factory):
+class Build:
+    """A mock host with the plugins it loads (and their library)."""
+
+    def __init__(self, host, lib_dir, env=None):
+        self.host = host
+        self.plugin = os.path.join(lib_dir, "libbm3dcuda_mi355x.so")
+        self.plugin_rtc = os.path.join(lib_dir, "libbm3dcuda_rtc_mi355x.so")
+        self.lib = os.path.join(lib_dir, "libbm3d_mi355x.so")
+        self.env = env or {}
+
+
+def _plain(tmp_path_factory):
     if not shutil.which("g++"):
         pytest.skip("g++ not available")
     if not os.path.exists(PLUGIN):
@@ -52,13 +62,44 @@ def host(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("vs_host") / "mock_vs_host")
     subprocess.run(["g++", "-O1", "-g", "-std=c++17", f"-I{os.path.join(PKG, 'vsplugin')}", HOST_SRC, "-o", exe,
                     "-ldl", "-pthread"], check=True)
-    return exe
+    return Build(exe, LIB)
+
+
+def _sanitized(tmp_path_factory):
+    """AddressSanitizer + UBSan build of the library's host code, the plugins
+    and the mock host (host code only; kernels as built)."""
+    import build_lib
+    if not os.path.exists(build_lib.CLANGXX):
+        pytest.skip("ROCm clang++ not available")
+    out = str(tmp_path_factory.mktemp("asan"))
+    exe = build_lib.build_sanitized(out)
+    return Build(exe, out, env={"ASAN_OPTIONS": "detect_leaks=1:exitcode=23",
+                                "UBSAN_OPTIONS": "print_stacktrace=1:halt_on_error=1"})
+
+
+@pytest.fixture(scope="module")
+def host(tmp_path_factory):
+    return _plain(tmp_path_factory)
+
+
+@pytest.fixture(scope="module", params=["plain", "asan+ubsan"])
+def cpu_build(request, tmp_path_factory):
+    """The CPU tests run on the plain build and on the sanitizer build."""
+    if request.param == "plain":
+        return _plain(tmp_path_factory)
+    if not os.environ.get("BM3D_SKIP_SANITIZED"):
+        return _sanitized(tmp_path_factory)
+    pytest.skip("sanitizer build skipped (BM3D_SKIP_SANITIZED)")
 
 
 def run(host, script, env=None, timeout=240):
+    if isinstance(host, Build):
+        env = dict(host.env, **(env or {}))
+        host = host.host
     r = subprocess.run([host], input=script, capture_output=True, text=True, timeout=timeout,
                        env=dict(os.environ, **(env or {})))
     assert r.returncode == 0, f"host exited {r.returncode}\n{r.stdout}\n{r.stderr}"
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr
     assert r.stdout.rstrip().endswith("done"), r.stdout
     return r.stdout.splitlines()
 
@@ -85,9 +126,10 @@ def errors_of(lines):
 
 
 # ---------------------------------------------------------------- CPU
-def test_registration(host):
-    for lib, ns, args in ((PLUGIN, "bm3dcuda", REF_ARGS), (PLUGIN_RTC, "bm3dcuda_rtc", REF_RTC_ARGS)):
-        lines = run(host, f"plugin {lib}\n")
+def test_registration(cpu_build):
+    b = cpu_build
+    for lib, ns, args in ((b.plugin, "bm3dcuda", REF_ARGS), (b.plugin_rtc, "bm3dcuda_rtc", REF_RTC_ARGS)):
+        lines = run(b, f"plugin {lib}\n")
         cfg = lines[0].split(" ", 5)
         assert cfg[:5] == ["config", f"com.wolframrhodium.{ns}", ns, str(API_VERSION), "1"]
         reg = {l.split(" ")[1]: l.split(" ")[2] for l in lines if l.startswith("register ")}
@@ -96,7 +138,7 @@ def test_registration(host):
         assert reg["VAggregate"] == VAGG_ARGS
 
 
-def test_creation_errors(host, tmp_path):
+def test_creation_errors(cpu_build, tmp_path):
     """The reference's checks, in its order, before any device is touched."""
     g = write_raw(tmp_path / "g.raw", np.zeros((3, 1, 40, 48), np.float32))
     g2 = write_raw(tmp_path / "g2.raw", np.zeros((2, 1, 40, 48), np.float32))
@@ -121,7 +163,7 @@ def test_creation_errors(host, tmp_path):
         ("clip=n:g precision=s:fp16", 'BM3D: "precision" must be "fp32" or "mixed"'),
         ("clip=n:g device_id=i:-1", "BM3D: invalid device ID (-1)"),
     ]
-    script = [f"plugin {PLUGIN}",
+    script = [f"plugin {cpu_build.plugin}",
               f"source g GRAYS 48 40 3 {g}", f"source g2 GRAYS 48 40 2 {g2}",
               f"source gw GRAYS 40 40 3 {write_raw(tmp_path / 'gw.raw', np.zeros((3, 1, 40, 40)))}",
               f"source y YUV444PS 48 40 3 {yuv}", f"source y420 YUV420PS 48 40 3 {y420}",
@@ -134,17 +176,21 @@ def test_creation_errors(host, tmp_path):
     script += [f"invoke out BM3Dv2 {args}" for args, _ in v2]
     script += ["invoke v VAggregate clip=n:g src=n:g planes=i:0",
                "invoke v VAggregate clip=n:g src=n:g planes=i:3"]
-    got = errors_of(run(host, "\n".join(script) + "\n"))
+    got = errors_of(run(cpu_build, "\n".join(script) + "\n"))
     want = [m for _, m in cases] + [m for _, m in v2]
     want += ['VAggregate: "clip" is not a BM3D output with radius > 0 for "src"'] * 2
     assert got == want
 
 
-def test_bm3dv2_passthrough(host, tmp_path):
-    """BM3Dv2 with every sigma below FLT_EPSILON returns the clip itself."""
-    g = write_raw(tmp_path / "g.raw", np.zeros((2, 1, 16, 16), np.float32))
-    lines = run(host, f"plugin {PLUGIN}\nsource s GRAYS 16 16 2 {g}\ninvoke o BM3Dv2 clip=n:s sigma=f:0\n")
+def test_bm3dv2_passthrough(cpu_build, tmp_path):
+    """BM3Dv2 with every sigma below FLT_EPSILON returns the clip itself, and
+    its frames are the source's."""
+    g = write_raw(tmp_path / "g.raw", np.arange(2 * 16 * 16, dtype=np.float32).reshape(2, 1, 16, 16))
+    lines = run(cpu_build, f"plugin {cpu_build.plugin}\nsource s GRAYS 16 16 2 {g}\n"
+                           f"invoke o BM3Dv2 clip=n:s sigma=f:0\nfetch o {tmp_path / 'o.raw'} 2\n")
     assert "ok o 16 16 2 source" in lines
+    (o,) = read_raw(tmp_path / "o.raw", 2, [(16, 16)])
+    np.testing.assert_array_equal(o, np.fromfile(g, np.float32).reshape(2, 16, 16))
 
 
 # ---------------------------------------------------------------- GPU

@@ -87,13 +87,59 @@ PLUGINS = [
 ]
 
 
-def plugin_cmd(out, extra, sanitize=False):
-    """g++ line of one plugin entry (host code only: it calls the C ABI)."""
+def plugin_cmd(out, extra, compiler=None, sanitize=False):
+    """Compile line of one plugin entry (host code only: it calls the C ABI)."""
     lib_dir = os.path.dirname(out)
-    san = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-g"] if sanitize else []
-    return [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", *san, *extra,
+    san = SANITIZE if sanitize else []
+    return [compiler or CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", *san, *extra,
             f"-I{os.path.join(ROOT, 'include')}", f"-I{VSPLUGIN}", os.path.join(VSPLUGIN, "bm3d_vsplugin.cpp"),
             f"-L{lib_dir}", "-lbm3d_mi355x", "-Wl,-rpath,$ORIGIN", "-o", out]
+
+
+CLANGXX = "/opt/rocm/llvm/bin/clang++"
+SANITIZE = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(out_dir, jobs=4):
+    """AddressSanitizer + UBSan build of the host code, into out_dir: the
+    library with its host-only sources instrumented (device code and the
+    kernel launchers stay as built by build()), both plugin entries, and
+    the mock VapourSynth host of the tests. Every sanitizer flag applies to
+    the host pass only. Returns the path of the mock host executable."""
+    if not os.path.exists(LIB):
+        build(jobs)
+    os.makedirs(out_dir, exist_ok=True)
+    host_flags = [f for flag in SANITIZE for f in ("-Xarch_host", flag)]
+
+    def compile_san(src):
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *COMMON, *DEVICE, *host_flags, "-x", "hip", "-c", src, "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return obj
+
+    host_srcs = [s for s in sources() if s.endswith(".cpp")]
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(compile_san, host_srcs))
+    objs += [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in sources() if s.endswith(".hip")]
+    lib = os.path.join(out_dir, "libbm3d_mi355x.so")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    for name, extra in PLUGINS:
+        cmd = plugin_cmd(os.path.join(out_dir, name), extra, compiler=CLANGXX, sanitize=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"plugin build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    host = os.path.join(out_dir, "mock_vs_host")
+    cmd = [CLANGXX, "-O1", "-std=c++17", *SANITIZE, f"-I{VSPLUGIN}",
+           os.path.join(ROOT, "tests", "vs_host", "mock_vs_host.cpp"), "-o", host, "-ldl", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mock host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return host
 
 
 def build_plugins():
