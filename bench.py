@@ -138,7 +138,7 @@ def cpu_baseline(noisy, basic, threads, frames=(4, 9, 14, 19, 24, 27, 10, 21)):
 class Pipeline:
     """Two-pass BM3Dv2 (radius 1) over a device-resident clip."""
 
-    def __init__(self, core, src, precision):
+    def __init__(self, core, src, precision, spatial_mfma=False):
         self.core = core
         self.src = src
         n, _, h, w = src.shape
@@ -148,6 +148,10 @@ class Pipeline:
                             p["ps_num"], p["ps_range"], final=False, precision=precision)
         self.pf = core.Plan(w, h, p["sigma"], p["block_step"], p["bm_range"], p["radius"],
                             p["ps_num"], p["ps_range"], final=True, precision=precision)
+        if spatial_mfma and precision == "mixed":
+            # the opt-in matcher whose spatial search runs on the matrix cores
+            self.pb.select_kernels("mixed-mfma", "mixed")
+            self.pf.select_kernels("mixed-mfma", "mixed")
         import torch
         self.basic = torch.empty_like(src)
         self.final = torch.empty_like(src)
@@ -165,9 +169,9 @@ class Pipeline:
         self.pf.execute_vagg(self.src, self.basic, out=self.final, stream=stream)
 
 
-def time_path(core, capi, src, precision, steps, warmup, autotune):
+def time_path(core, capi, src, precision, steps, warmup, autotune, spatial_mfma=False):
     import torch
-    pipe = Pipeline(core, src, precision)
+    pipe = Pipeline(core, src, precision, spatial_mfma)
     tuning = pipe.autotune() if autotune else {"basic": pipe.pb.tuning, "final": pipe.pf.tuning}
     log(f"{precision}: launch shapes {tuning}")
     stream = torch.cuda.current_stream()
@@ -479,6 +483,8 @@ def main():
                     help="after the run, fill and copy a buffer of this many MiB (a known-size "
                          "reference for HBM byte counters)")
     ap.add_argument("--no-compare", action="store_true", help="time only --precision")
+    ap.add_argument("--spatial-mfma", action="store_true",
+                    help="mixed path: the matcher's spatial search on the matrix cores (opt-in kernel)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the plans' default launch shapes instead of autotuned ones")
     ap.add_argument("--workload", default="vbm3d", choices=["vbm3d", "batch", "host"],
@@ -516,7 +522,8 @@ def main():
     src = torch.from_numpy(noisy).cuda()
 
     paths = [args.precision] if args.no_compare else ["fp32", "mixed"]
-    res = {p: time_path(core, _capi, src, p, args.steps, args.warmup, not args.no_autotune) for p in paths}
+    res = {p: time_path(core, _capi, src, p, args.steps, args.warmup, not args.no_autotune, args.spatial_mfma)
+           for p in paths}
     main_r = res[args.precision]
 
     basic_host = main_r["pipe"].basic.cpu().numpy()

@@ -150,16 +150,23 @@ class Plan:
         """True when block matching runs on the mixed-precision kernel."""
         return bool(C.lib().bm3d_plan_matching_kernel(self._h))
 
+    def uses_matrix_core_search(self):
+        """True when the mixed kernel's spatial search may use the matrix cores."""
+        return C.lib().bm3d_plan_matching_kernel(self._h) == 2
+
     def uses_mixed_filter(self):
         """True when collaborative filtering runs on the mixed-precision kernel."""
         return bool(C.lib().bm3d_plan_filter_kernel(self._h))
 
     def select_kernels(self, matching, filter):
-        """Selects the matching and filter kernels: each "fp32" or "mixed"."""
+        """Selects the matching and filter kernels: each "fp32" or "mixed";
+        matching "mixed-mfma" is the mixed kernel with its spatial search's
+        cross terms on the matrix cores."""
         codes = {"fp32": 0, "mixed": 1}
-        if matching not in codes or filter not in codes:
-            raise ValueError("kernel must be 'fp32' or 'mixed'")
-        C.check(C.lib().bm3d_plan_select_kernels(self._h, codes[matching], codes[filter]))
+        mcodes = dict(codes, **{"mixed-mfma": 2})
+        if matching not in mcodes or filter not in codes:
+            raise ValueError("kernel must be 'fp32' or 'mixed' (matching also 'mixed-mfma')")
+        C.check(C.lib().bm3d_plan_select_kernels(self._h, mcodes[matching], codes[filter]))
 
     @property
     def tuning(self):

@@ -275,7 +275,9 @@ void dx_geometry(bm3d_plan* p) {
     if (nWs * nWs > 4096 || (d.radius > 0 && d.ps_num * nWt * nWt > 4096)) return;
     p->tile_hp2 = ((p->tile_h + 1) / 2 + 1) | 1;
     const size_t bytes = (size_t)2 * d.block_step * p->tile_wq * p->tile_hp2 * 4;
-    if (bytes + 8192 > (size_t)p->max_lds) return;  // + the kernel's static list buffer
+    // + the kernel's static arrays (11 KB: the spatial list, the matrix-core
+    // search's column sums)
+    if (bytes + 12288 > (size_t)p->max_lds) return;
     p->dx_lds_bytes = bytes;
     p->dx_ok = true;
 }
@@ -407,6 +409,7 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.tile_hp2 = p->tile_hp2;
     m.cost = d.bm_error;
     m.kt = bm3d::plan_temporal_keys(p);
+    m.mx_spatial = p->mx_spatial ? 1 : 0;
     return m;
 }
 
@@ -901,15 +904,16 @@ int bm3d_profile_read(int kernel, double* total_ms, long long* launches) {
     return BM3D_OK;
 }
 
-int bm3d_plan_matching_kernel(const bm3d_plan* p) { return p && p->use_dx ? 1 : 0; }
+int bm3d_plan_matching_kernel(const bm3d_plan* p) { return p && p->use_dx ? (p->mx_spatial ? 2 : 1) : 0; }
 
 int bm3d_plan_filter_kernel(const bm3d_plan* p) { return p && p->use_mx_filter ? 1 : 0; }
 
 int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
     g_last_error.clear();
     if (!p) return fail(BM3D_EINVAL, "BM3D: null argument");
-    if (matching < 0 || matching > 1 || filter < 0 || filter > 1)
-        return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision)");
+    if (matching < 0 || matching > 2 || filter < 0 || filter > 1)
+        return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision), or 2 for the "
+                                 "mixed matcher with its spatial search on the matrix cores");
     if ((matching || filter) && p->d.precision != BM3D_PRECISION_MIXED)
         return fail(BM3D_EINVAL, "BM3D: mixed-precision kernels need a plan created with precision MIXED");
     if (matching && p->d.bm_error != BM3D_BM_SSD)
@@ -917,6 +921,7 @@ int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
     if (matching && !p->dx_ok)
         return fail(BM3D_EUNSUPPORTED, "BM3D: search windows too large for the mixed-precision matching kernel");
     p->use_dx = matching != 0;
+    p->mx_spatial = matching == 2;
     p->use_mx_filter = filter != 0;
     return BM3D_OK;
 }

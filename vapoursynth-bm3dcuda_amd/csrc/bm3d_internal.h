@@ -66,6 +66,7 @@ struct bm3d_plan {
     bool pinned = false;
     // mixed-precision matching kernel (precision MIXED)
     bool use_dx = false, dx_ok = false;
+    bool mx_spatial = false;  // the mixed matcher's spatial search on the matrix cores (where eligible; opt-in)
     int tile_hp2 = 0;
     size_t dx_lds_bytes = 0;
     // mixed-precision filter kernel (precision MIXED): per-lane transform operands

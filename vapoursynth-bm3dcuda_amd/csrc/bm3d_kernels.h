@@ -49,6 +49,7 @@ struct MatchArgs {
     unsigned long long* ncand;  // non-null: count in-plane candidates
     int cost;                // block distance, 0 .. kNumCosts - 1 (the mixed kernel: 0 only)
     int kt;                  // mixed kernel: keys kept per temporal frame (4, 8 or 16)
+    int mx_spatial;          // mixed kernel: spatial search on the matrix cores where eligible
 };
 
 struct FilterArgs {

@@ -366,6 +366,218 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
         dx_rows<K, kCount, true, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
 }
 
+// ---------------------------------------------------------------------------
+// Spatial search with the cross term on the matrix cores.
+//
+// A wave holds 4 rows (groups) of 16 reference blocks, bs apart; lane
+// (q, c) = (lane / 16, lane % 16) owns block c of group q. For one candidate
+// row dy of group g, every block's candidates lie in one union row of
+// S = 15 bs + 2 R + 1 positions, cut into T <= 5 tiles of 16. Per tile, two
+// v_mfma_f32_16x16x32_f16 products give D = -<r'_m, c'_s> for the group's 16
+// blocks m and the tile's 16 positions s, with K = 64 pixels ordered
+// (column, row): A (lane (q, c): block c of the group, columns q and q + 4,
+// rows 0..7) and B (lane (q, c): position c of the tile, the same columns)
+// are the FP16 vertical pairs of the staged tile, so B is four words of a
+// tile column, as the VALU scan loads them. Lane (q, c) receives D for
+// blocks 4 q + i and position c; the values of a block's window (position
+// bs m + dxi, dxi in [0, 2R]) go through LDS to lanes (dxi % 4, m), which
+// form the keys (N = |c'|^2 from a per-row table of column sums) and keep
+// one partial top-K per group. At the end a 4 x 4 lane-group transpose
+// (v_permlane32_swap, v_permlane16_swap) brings a block's four partial lists
+// to its owner, which merges them.
+//
+// Keys equal the VALU scan's up to FP32 summation order: the same FP16
+// residuals, exact FP16 products, FP32 sums, the same packing and scan
+// indices. Requires 16 blocks per row, all bs apart inside the plane,
+// bs | 16, S <= 80 and 2 R + 1 <= 28. Opt-in (MatchArgs::mx_spatial): on
+// MI355X it issues 13% fewer VALU instructions than the VALU scan but about
+// three times the LDS traffic and waits (the exchange and the |c'|^2 table),
+// and runs a few percent slower; 76% of each product is outside the blocks'
+// windows.
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMxTiles = 5;
+constexpr int kMxMaxP = 28;       // 2 R + 1
+constexpr int kMxExWords = 512;   // exchange words per wave: 16 P + 64 spill slots
+constexpr int kMxCnWords = 176;   // column sums (S + 7) and N (S) per wave
+
+__device__ __forceinline__ bool mx_eligible(const MatchArgs& a, int bx0) {
+    const int R = a.bm_range, P = 2 * R + 1, S = 15 * a.bs + P;
+    return a.mx_spatial && a.tbw == 16 && (16 % a.bs) == 0 && S <= 16 * kMxTiles && P <= kMxMaxP &&
+           bx0 + 15 < a.nbx && (bx0 + 15) * a.bs <= a.W - 8;
+}
+
+// Merges sorted list b into the K smallest of a and b (a sorted ascending).
+template <int K>
+__device__ __forceinline__ void merge_sorted(KeyList<K>& a, const uint32_t (&b)[K]) {
+    constexpr int P = K <= 8 ? 8 : 16;
+    uint32_t c[P];
+#pragma unroll
+    for (int i = 0; i < P - K; ++i) c[i] = 0u;
+#pragma unroll
+    for (int i = 0; i < K; ++i) c[P - K + i] = min(a.k[i], b[K - 1 - i]);
+#pragma unroll
+    for (int d = P / 2; d >= 1; d /= 2)
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+            if ((i & d) == 0) cex(c[i], c[i + d]);
+#pragma unroll
+    for (int i = 0; i < K; ++i) a.k[i] = c[P - K + i];
+}
+
+// The spatial search of the centre slot for the 64 blocks of this wave
+// into kl (the lane's own block), from the staged tile at `lds`. ex: the
+// wave's 512 exchange words; cn: the wave's column-sum table. Groups are
+// the outer loop, so only one group's A operands are live at a time.
+template <bool kCount>
+__device__ __forceinline__ void mx_window(const uint32_t* __restrict__ lds, const PairGeom& pg, const MatchArgs& a,
+                                          const float* __restrict__ plane, float mu, int x0, const int (&yg)[4],
+                                          const bool (&row_on)[4], uint32_t* __restrict__ ex,
+                                          float* __restrict__ cn, KeyList<kKSpatial>& kl, unsigned& cnt) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+    const int bs = a.bs, R = a.bm_range, P = 2 * R + 1, S = 15 * bs + P;
+    const int T = (S + 15) >> 4, J = (P + 3) >> 2;
+    const int cbase = a.margin - R;  // tile column of union position 0, column 0
+
+    // exchange slots: writer (tile t, register i) -> block 4 q + i, window
+    // column dxi = 16 t + c - bs (4 q + i), or the lane's spill slot
+    int wr[kMxTiles][4];
+#pragma unroll
+    for (int t = 0; t < kMxTiles; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 4 * q + i, dxi = 16 * t + c - bs * m;
+            wr[t][i] = (dxi >= 0 && dxi < P) ? m * P + dxi : 16 * P + lane;
+        }
+    // reader: lane (q, c) takes window columns dxi = q + 4 j of block c
+    const int rd = c * P + q;
+    unsigned col_ok = 0;  // bit j: dxi in the window and the candidate column in the plane
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const int dxi = q + 4 * j, cx = x0 + bs * c - R + dxi;
+        col_ok |= (j < J && dxi < P && cx >= 0 && cx <= a.W - 8) ? 1u << j : 0u;
+    }
+    // B operand columns: union position 16 t + c, columns q and q + 4; the
+    // last tile's positions are clamped to the union (beyond it they feed
+    // only spill slots)
+    const int tstep = (16 / bs) * pg.hp2;  // 16 columns on: the same phase
+    const int cb0 = pair_col_base(pg, cbase + c + q), cb1 = pair_col_base(pg, cbase + c + q + 4);
+    const int sl = min(16 * (T - 1) + c, S - 1);
+    const int cl0 = pair_col_base(pg, cbase + sl + q), cl1 = pair_col_base(pg, cbase + sl + q + 4);
+    // columns of the column-sum table this lane fills (lane, lane + 64)
+    const int cs0 = pair_col_base(pg, cbase + min(lane, S + 6));
+    const int cs1 = pair_col_base(pg, cbase + min(lane + 64, S + 6));
+    uint32_t hi_mask;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(hi_mask) : "i"(~kIdxMask));
+
+    KeyList<kKSpatial> L[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        L[g].init();
+        // A operands and |r'|^2 / 2 of block c of the group (FP16 residuals
+        // as the staged tile holds them, negated: D accumulates -<r', c'>)
+        h8 A0, A1;
+        float nr;
+        {
+            const float* blk = plane + (size_t)yg[g] * a.stride + x0 + bs * c + q;
+            float part = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                A0[j] = -(_Float16)(blk[(size_t)j * a.stride] - mu);
+                A1[j] = -(_Float16)(blk[(size_t)j * a.stride + 4] - mu);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const f16x2 p0{A0[j], A0[j + 1]}, p1{A1[j], A1[j + 1]};
+                part = dot2(p1, p1, dot2(p0, p0, part));
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            nr = 0.5f * part;
+        }
+        for (int dyi = 0; dyi < P; ++dyi) {
+            const int cy = yg[g] - R + dyi;
+            const int rt = cy - pg.ty0;
+            const uint32_t* tile = lds + (rt & 1) * pg.plane + (rt >> 1);
+            // column sums of squares over the 8 candidate rows, then N per position
+            if (lane < S + 7) {
+                f16x2 w[4];
+                load_pair_col(w, tile + cs0);
+                cn[lane] = col_sq(w);
+                if (lane + 64 < S + 7) {
+                    load_pair_col(w, tile + cs1);
+                    cn[lane + 64] = col_sq(w);
+                }
+            }
+            for (int s = lane; s < S; s += 64) {
+                float nsum = cn[s];
+#pragma unroll
+                for (int u = 1; u < 8; ++u) nsum += cn[s + u];
+                cn[kMxCnWords - 80 + s] = nsum;
+            }
+            // cross terms, tile by tile, into the exchange slots
+#pragma unroll
+            for (int t = 0; t < kMxTiles; ++t) {
+                if (t >= T) break;  // uniform
+                const bool last = t == T - 1;
+                const uint32_t* p0 = tile + (last ? cl0 : cb0 + t * tstep);
+                const uint32_t* p1 = tile + (last ? cl1 : cb1 + t * tstep);
+                h8 B0, B1;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const f16x2 v0 = as_h2(p0[w]), v1 = as_h2(p1[w]);
+                    B0[2 * w] = v0[0]; B0[2 * w + 1] = v0[1];
+                    B1[2 * w] = v1[0]; B1[2 * w + 1] = v1[1];
+                }
+                f4 D = {0.0f, 0.0f, 0.0f, 0.0f};
+                D = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, B0, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, B1, D, 0, 0, 0);
+                // through scalars (a bit cast of the vector element D[i]
+                // compiled to four stores of element 0)
+                const float d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
+                ex[wr[t][0]] = __float_as_uint(d0);
+                ex[wr[t][1]] = __float_as_uint(d1);
+                ex[wr[t][2]] = __float_as_uint(d2);
+                ex[wr[t][3]] = __float_as_uint(d3);
+            }
+            // keys of block c of the group, window columns q + 4 j
+            const bool row_in = row_on[g] && cy >= 0 && cy <= a.H - 8;
+            const uint32_t rowidx = (uint32_t)(dyi * P);
+            uint32_t keys[8];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int dxi = q + 4 * j;
+                const float d = __uint_as_float(ex[min(rd + 4 * j, 16 * P - 1)]);
+                const float nsum = cn[kMxCnWords - 80 + min(bs * c + dxi, S - 1)];
+                const float approx = fmaxf(fma_(nsum, 0.5f, nr) + d, 0.0f);
+                const bool ok = row_in && ((col_ok >> j) & 1u);
+                if constexpr (kCount) cnt += ok ? 1u : 0u;
+                keys[j] = ok ? ((__builtin_bit_cast(uint32_t, approx) & hi_mask) | (rowidx + (uint32_t)dxi))
+                             : 0xFFFFFFFFu;
+            }
+            keys[7] = 0xFFFFFFFFu;
+            L[g].push8(keys);
+        }
+    }
+
+    // lane group g collects the four partial lists of its own blocks
+#pragma unroll
+    for (int k = 0; k < kKSpatial; ++k) {
+        auto s02 = __builtin_amdgcn_permlane32_swap(L[0].k[k], L[2].k[k], false, false);
+        auto s13 = __builtin_amdgcn_permlane32_swap(L[1].k[k], L[3].k[k], false, false);
+        L[0].k[k] = s02[0]; L[2].k[k] = s02[1]; L[1].k[k] = s13[0]; L[3].k[k] = s13[1];
+        auto s01 = __builtin_amdgcn_permlane16_swap(L[0].k[k], L[1].k[k], false, false);
+        auto s23 = __builtin_amdgcn_permlane16_swap(L[2].k[k], L[3].k[k], false, false);
+        L[0].k[k] = s01[0]; L[1].k[k] = s01[1]; L[2].k[k] = s23[0]; L[3].k[k] = s23[1];
+    }
+    kl = L[0];
+    merge_sorted(kl, L[1].k);
+    merge_sorted(kl, L[2].k);
+    merge_sorted(kl, L[3].k);
+}
+
 // Exact distance between the block at (x, y) of `rplane` and the candidate
 // at (cx, cy) of `plane`, in the order of cpu_source/source.cpp:123-136 +
 // 98-103 (as ring_ssd); rows read from memory as two 16-byte loads each.
@@ -449,7 +661,8 @@ template <bool kCount, int kKt, bool kDiv8>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
-    __shared__ uint32_t centre_s[8][256];  // the spatial list, per lane
+    __shared__ uint32_t centre_s[256][8];  // the spatial list, per lane (before it: mx_window's exchange)
+    __shared__ float mx_cn[4][kMxCnWords];  // mx_window's column sums, per wave
     const int wshift = a.tbw == 32 ? 5 : 4;
     const int tby = blockDim.x >> wshift;
     const int lx = threadIdx.x & (a.tbw - 1), ly = threadIdx.x >> wshift;
@@ -497,17 +710,33 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
         return stage_pairs(lds, pg, img, a.W, a.H, a.stride, mu);
     };
 
-    unsigned cnt = 0;
+    unsigned cnt = 0, cnt_wave = 0;  // cnt_wave: counted for other lanes' blocks too
     Best8 best;
     {
         float mu;
         const bool fp16 = stage(plane_r, mu);
         if (fp16) {
-            f16x2 rp[8][4];
-            const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
             KeyList<kKSpatial> kl;
             kl.init();
-            dx_window<kKSpatial, kCount, kDiv8>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
+            bool mx = false;
+            if constexpr (kDiv8) mx = mx_eligible(a, bx0) && (tby & 3) == 0;
+            if (mx) {
+                const int wv = threadIdx.x >> 6;
+                int yg[4];
+                bool row_on[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int row = by0 + 4 * wv + g;
+                    yg[g] = min(min(row, a.nby - 1) * a.bs, a.H - 8);
+                    row_on[g] = row < a.nby;
+                }
+                mx_window<kCount>(lds, pg, a, plane_r, mu, bx0 * a.bs, yg, row_on,
+                                  &centre_s[64 * wv][0], mx_cn[wv], kl, cnt_wave);
+            } else {
+                f16x2 rp[8][4];
+                const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
+                dx_window<kKSpatial, kCount, kDiv8>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
+            }
             Best8Id bi;
             bi.init(kFltMax, pack_pos(x, y, r));
             const int nW = 2 * a.bm_range + 1;
@@ -532,12 +761,12 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
 
     if (r > 0) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) centre_s[i][threadIdx.x] = best.p[i];
+        for (int i = 0; i < 8; ++i) centre_s[threadIdx.x][i] = best.p[i];
         const int nWt = 2 * a.ps_range + 1;
         for (int dir = -1; dir <= 1; dir += 2) {
             uint32_t last[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) last[i] = centre_s[i][threadIdx.x];
+            for (int i = 0; i < 8; ++i) last[i] = centre_s[threadIdx.x][i];
             for (int t = 1; t <= r; ++t) {
                 const int z = r + dir * t;
                 const float* plane_z = slot_plane(z);
@@ -609,7 +838,8 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
         best.p[0] = cpos;
     }
     if constexpr (kCount) {
-        if (active) atomicAdd(a.ncand, (unsigned long long)cnt);
+        const unsigned total = (active ? cnt : 0u) + cnt_wave;
+        if (total) atomicAdd(a.ncand, (unsigned long long)total);
     }
     if (active && a.groups) {
         uint4* out = reinterpret_cast<uint4*>(a.groups + ((size_t)f * a.nblocks + (size_t)byi * a.nbx + bxi) * 8);
