@@ -128,10 +128,13 @@ def errors_of(lines):
 # ---------------------------------------------------------------- CPU
 def test_registration(cpu_build):
     b = cpu_build
-    for lib, ns, args in ((b.plugin, "bm3dcuda", REF_ARGS), (b.plugin_rtc, "bm3dcuda_rtc", REF_RTC_ARGS)):
+    own = os.path.join(os.path.dirname(b.plugin), "libbm3dmi355x.so")
+    for lib, ident, ns, args in ((b.plugin, "com.wolframrhodium.bm3dcuda", "bm3dcuda", REF_ARGS),
+                                 (b.plugin_rtc, "com.wolframrhodium.bm3dcuda_rtc", "bm3dcuda_rtc", REF_RTC_ARGS),
+                                 (own, "com.mi355x.bm3d", "bm3dmi355x", REF_RTC_ARGS)):
         lines = run(b, f"plugin {lib}\n")
         cfg = lines[0].split(" ", 5)
-        assert cfg[:5] == ["config", f"com.wolframrhodium.{ns}", ns, str(API_VERSION), "1"]
+        assert cfg[:5] == ["config", ident, ns, str(API_VERSION), "1"]
         reg = {l.split(" ")[1]: l.split(" ")[2] for l in lines if l.startswith("register ")}
         assert list(reg) == ["BM3D", "VAggregate", "BM3Dv2"]
         assert reg["BM3D"] == reg["BM3Dv2"] == args + "precision:data:opt;"

@@ -80,10 +80,12 @@ VSPLUGIN = os.path.join(HERE, "vsplugin")
 CXX = os.environ.get("CXX", "g++")
 # VapourSynth plugin entries over the library: (output, extra flags). The
 # first serves the bm3dcuda namespace, the second bm3dcuda_rtc (the NVRTC
-# variant's string options).
+# variant's string options), the third a namespace of its own (bm3dmi355x)
+# that can be installed beside either.
 PLUGINS = [
     ("libbm3dcuda_mi355x.so", []),
     ("libbm3dcuda_rtc_mi355x.so", ["-DBM3D_VS_RTC"]),
+    ("libbm3dmi355x.so", ["-DBM3D_VS_OWN_NAMESPACE"]),
 ]
 
 

@@ -2,8 +2,9 @@
 // VapourSynthPluginInit registers BM3D, VAggregate and BM3Dv2 with the
 // reference's identifiers and argument strings (source/source.cpp:991-1027;
 // built with BM3D_VS_RTC, those of the NVRTC variant,
-// rtc_source/source.cpp:1358-1397), and the filters run on libbm3d_mi355x
-// through its C ABI:
+// rtc_source/source.cpp:1358-1397; with BM3D_VS_OWN_NAMESPACE, the NVRTC
+// variant's arguments under an identifier and namespace of its own), and the
+// filters run on libbm3d_mi355x through its C ABI:
 //   * creation parses the argument map into bm3d_args and calls
 //     bm3d_filter_create / bm3dv2_filter_create / bm3d_vaggregate_create,
 //     which apply the reference's defaults, checks and messages;
@@ -27,7 +28,17 @@
 
 namespace {
 
-#ifdef BM3D_VS_RTC
+#if defined(BM3D_VS_OWN_NAMESPACE)
+// its own identifier and namespace, to install beside a bm3dcuda plugin
+constexpr const char* kIdentifier = "com.mi355x.bm3d";
+constexpr const char* kNamespace = "bm3dmi355x";
+constexpr const char* kDescription = "BM3D algorithm implemented in HIP for MI355X";
+constexpr const char* kBm3dArgs =
+    "clip:clip;ref:clip:opt;sigma:float[]:opt;block_step:int[]:opt;bm_range:int[]:opt;radius:int:opt;"
+    "ps_num:int[]:opt;ps_range:int[]:opt;chroma:int:opt;device_id:int:opt;fast:int:opt;extractor_exp:int:opt;"
+    "bm_error_s:data[]:opt;transform_2d_s:data[]:opt;transform_1d_s:data[]:opt;zero_init:int:opt;"
+    "precision:data:opt;";
+#elif defined(BM3D_VS_RTC)
 constexpr const char* kIdentifier = "com.wolframrhodium.bm3dcuda_rtc";
 constexpr const char* kNamespace = "bm3dcuda_rtc";
 constexpr const char* kDescription = "BM3D algorithm implemented in HIP for MI355X (bm3dcuda_rtc interface)";
