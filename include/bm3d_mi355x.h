@@ -199,11 +199,14 @@ int bm3d_match(bm3d_plan *plan, unsigned int *d_groups, const bm3d_clip *src, co
 long long bm3d_num_blocks(const bm3d_plan *plan);
 
 /* Block-matching kernel of the plan: 1 for the mixed-precision kernel
- * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best few),
- * 2 for the same kernel with the spatial search's cross terms on the matrix
- * cores where the work-group tile allows (opt-in: slower on MI355X), 0 for
- * the FP32 kernel (precision FP32, or a search window too large for the
- * mixed kernel's 12-bit scan indices). No counterpart in the reference. */
+ * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best few)
+ * with a per-lane spatial search, 3 for the same kernel with its spatial
+ * search by strips shared between neighbouring lanes (the default for block
+ * steps 2 and 4 at radius 0), 2 for the same kernel with the spatial search's cross
+ * terms on the matrix cores where the work-group tile allows (opt-in: slower
+ * on MI355X), 0 for the FP32 kernel (precision FP32, or a search window too
+ * large for the mixed kernel's 12-bit scan indices). No counterpart in the
+ * reference. */
 int bm3d_plan_matching_kernel(const bm3d_plan *plan);
 
 /* Filtering kernel of the plan: 1 for the mixed-precision kernel (precision
@@ -215,7 +218,9 @@ int bm3d_plan_filter_kernel(const bm3d_plan *plan);
 /* Selects the plan's kernels, for testing and measurement (no counterpart in
  * the reference): `matching` and `filter` each 0 (FP32) or 1 (mixed
  * precision); `matching` 2 is the mixed kernel with its spatial search's
- * cross terms on the matrix cores. Mixed kernels need a plan created with precision MIXED
+ * cross terms on the matrix cores, 3 the mixed kernel with its strip search
+ * (where the block step and tile width allow it; the per-lane search
+ * elsewhere). Mixed kernels need a plan created with precision MIXED
  * (BM3D_EINVAL otherwise); the mixed matching kernel also needs search
  * windows within its limits (BM3D_EUNSUPPORTED otherwise). */
 int bm3d_plan_select_kernels(bm3d_plan *plan, int matching, int filter);

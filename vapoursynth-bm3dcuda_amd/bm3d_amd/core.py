@@ -154,18 +154,25 @@ class Plan:
         """True when the mixed kernel's spatial search may use the matrix cores."""
         return C.lib().bm3d_plan_matching_kernel(self._h) == 2
 
+    def uses_strip_search(self):
+        """True when the mixed kernel's spatial search runs by strips shared
+        between lanes (block steps 2 and 4; the default there)."""
+        return C.lib().bm3d_plan_matching_kernel(self._h) == 3
+
     def uses_mixed_filter(self):
         """True when collaborative filtering runs on the mixed-precision kernel."""
         return bool(C.lib().bm3d_plan_filter_kernel(self._h))
 
     def select_kernels(self, matching, filter):
         """Selects the matching and filter kernels: each "fp32" or "mixed";
-        matching "mixed-mfma" is the mixed kernel with its spatial search's
-        cross terms on the matrix cores."""
+        matching "mixed" is the mixed kernel with a per-lane spatial search,
+        "mixed-mfma" with its spatial search's cross terms on the matrix
+        cores, "mixed-strip" with its spatial search by strips shared between
+        lanes (where the block step allows; the default)."""
         codes = {"fp32": 0, "mixed": 1}
-        mcodes = dict(codes, **{"mixed-mfma": 2})
+        mcodes = dict(codes, **{"mixed-mfma": 2, "mixed-strip": 3})
         if matching not in mcodes or filter not in codes:
-            raise ValueError("kernel must be 'fp32' or 'mixed' (matching also 'mixed-mfma')")
+            raise ValueError("kernel must be 'fp32' or 'mixed' (matching also 'mixed-mfma' or 'mixed-strip')")
         C.check(C.lib().bm3d_plan_select_kernels(self._h, mcodes[matching], codes[filter]))
 
     @property

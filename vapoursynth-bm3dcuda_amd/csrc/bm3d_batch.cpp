@@ -61,7 +61,7 @@ struct Launch {
     int block = 0;  // threads (matching) or waves (filtering)
     int keys = 0, final_ = 0, chroma = 0;  // keys: the mixed matcher's temporal keys
     int any_extractor = 0;                 // mixed filtering: some job has extractor_exp
-    int variant = 0;  // matching: block distance (FP32) or block step divides 8 (mixed);
+    int variant = 0;  // matching: block distance (FP32) or dx_spatial_variant (mixed);
                       // filtering: the FP32 generic-transform variant
     size_t lds = 0;
     size_t args_off = 0, grids_off = 0;  // in the table buffer
@@ -213,14 +213,13 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                 }
             }
             // block matching, grouped by kernel variant and block size (the
-            // variant: the distance for the FP32 matcher, the block step's
-            // divisibility for the mixed one)
+            // variant: the distance for the FP32 matcher, the spatial search
+            // for the mixed one)
             std::map<std::tuple<int, int, int, int>, std::vector<int>> mgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int keys = p->use_dx ? bm3d::plan_temporal_keys(p) : 0;
-                const int variant = p->use_dx ? (bm3d::dx_block_step_divides_8(p->d.block_step) ? 1 : 0)
-                                              : p->d.bm_error;
+                const int variant = p->use_dx ? bm3d::plan_dx_variant(p) : p->d.bm_error;
                 mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, variant}].push_back(i);
             }
             for (auto& [key, ids] : mgroups) {
@@ -238,7 +237,8 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     bm3d::MatchArgs a = bm3d::plan_match_args(p, &j.job.src, &ref);
                     a.groups = reinterpret_cast<uint32_t*>(base + j.groups_off);
                     a.first_frame = 0;
-                    const int gx = (p->nbx + p->tbw - 1) / p->tbw, gy = (p->nby + p->tby - 1) / p->tby;
+                    const int obw = bm3d::plan_match_owned_width(p);
+                    const int gx = (p->nbx + obw - 1) / obw, gy = (p->nby + p->tby - 1) / p->tby;
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy * j.job.src.num_frames;
                     l.lds = std::max(l.lds, p->use_dx ? p->dx_lds_bytes : p->lds_bytes);
@@ -355,7 +355,7 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                             l.wgs, l.block, l.lds, l.variant, s);
         case 2:
             return bm3d::launch_match_dx_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
-                                               l.wgs, l.block, l.lds, l.keys, l.variant != 0, s);
+                                               l.wgs, l.block, l.lds, l.keys, l.variant, s);
         case 3:
             return bm3d::launch_filter_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids, n,
                                              l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);

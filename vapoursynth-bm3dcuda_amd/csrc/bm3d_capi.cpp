@@ -410,6 +410,8 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.cost = d.bm_error;
     m.kt = bm3d::plan_temporal_keys(p);
     m.mx_spatial = p->mx_spatial ? 1 : 0;
+    m.strip = bm3d::plan_uses_strip(p) ? 1 : 0;
+    m.obw = bm3d::plan_match_owned_width(p);
     return m;
 }
 
@@ -664,6 +666,20 @@ const char* plan_tuning_error(const bm3d_tuning& t) { return tuning_range_error(
 int plan_temporal_keys(const bm3d_plan* p) {
     return p->d.mixed_temporal_keys ? p->d.mixed_temporal_keys : dx_temporal_keys(p->d.ps_num);
 }
+bool plan_uses_strip(const bm3d_plan* p) {
+    return p->use_dx && p->strip && !p->mx_spatial && p->lds_bytes && strip_supported(p->tbw, p->d.block_step);
+}
+int plan_match_owned_width(const bm3d_plan* p) {
+    return plan_uses_strip(p) ? strip_owned_width(p->tbw, p->d.block_step) : p->tbw;
+}
+int plan_dx_variant(const bm3d_plan* p) {
+    MatchArgs m{};
+    m.bs = p->d.block_step;
+    m.tbw = p->tbw;
+    m.strip = plan_uses_strip(p) ? 1 : 0;
+    m.mx_spatial = p->mx_spatial ? 1 : 0;
+    return dx_spatial_variant(m);
+}
 MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
     return match_args(p, nullptr, nullptr, src, ref, 0);
 }
@@ -773,6 +789,11 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         return rc;
     }
     p->use_dx = p->dx_ok;
+    // the strip search shortens the spatial search by about a fifth, but its
+    // last lanes per row own no block and idle through a temporal search: at
+    // radius 1 on the 1080p benchmark clip it was 5% slower than the per-lane
+    // search, at radius 0 17% faster
+    p->strip = d.radius == 0;
     p->vbound = bm3d::contribution_bound(d.transform_2d, d.transform_1d, p->d.final_ != 0);
     e = bm3d::malloc_device(&p->range, 256);  // the range, then the wrap flag
     if (e == hipSuccess) e = hipMemset(p->range, 0, 256);
@@ -904,16 +925,19 @@ int bm3d_profile_read(int kernel, double* total_ms, long long* launches) {
     return BM3D_OK;
 }
 
-int bm3d_plan_matching_kernel(const bm3d_plan* p) { return p && p->use_dx ? (p->mx_spatial ? 2 : 1) : 0; }
+int bm3d_plan_matching_kernel(const bm3d_plan* p) {
+    if (!p || !p->use_dx) return 0;
+    return p->mx_spatial ? 2 : bm3d::plan_uses_strip(p) ? 3 : 1;
+}
 
 int bm3d_plan_filter_kernel(const bm3d_plan* p) { return p && p->use_mx_filter ? 1 : 0; }
 
 int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
     g_last_error.clear();
     if (!p) return fail(BM3D_EINVAL, "BM3D: null argument");
-    if (matching < 0 || matching > 2 || filter < 0 || filter > 1)
-        return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision), or 2 for the "
-                                 "mixed matcher with its spatial search on the matrix cores");
+    if (matching < 0 || matching > 3 || filter < 0 || filter > 1)
+        return fail(BM3D_EINVAL, "BM3D: kernel selector must be 0 (FP32) or 1 (mixed precision), or for the mixed "
+                                 "matcher 2 (spatial search on the matrix cores) or 3 (spatial search by strips)");
     if ((matching || filter) && p->d.precision != BM3D_PRECISION_MIXED)
         return fail(BM3D_EINVAL, "BM3D: mixed-precision kernels need a plan created with precision MIXED");
     if (matching && p->d.bm_error != BM3D_BM_SSD)
@@ -922,6 +946,7 @@ int bm3d_plan_select_kernels(bm3d_plan* p, int matching, int filter) {
         return fail(BM3D_EUNSUPPORTED, "BM3D: search windows too large for the mixed-precision matching kernel");
     p->use_dx = matching != 0;
     p->mx_spatial = matching == 2;
+    p->strip = matching == 3;
     p->use_mx_filter = filter != 0;
     return BM3D_OK;
 }

@@ -67,6 +67,9 @@ struct bm3d_plan {
     // mixed-precision matching kernel (precision MIXED)
     bool use_dx = false, dx_ok = false;
     bool mx_spatial = false;  // the mixed matcher's spatial search on the matrix cores (where eligible; opt-in)
+    // the mixed matcher's spatial search by strips shared between lanes where
+    // the block step and tile allow it (strip_supported); the default at radius 0
+    bool strip = false;
     int tile_hp2 = 0;
     size_t dx_lds_bytes = 0;
     // mixed-precision filter kernel (precision MIXED): per-lane transform operands
@@ -91,6 +94,12 @@ MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_c
 // Keys the mixed matcher keeps per temporal frame: desc.mixed_temporal_keys,
 // or dx_temporal_keys(ps_num) when that is 0.
 int plan_temporal_keys(const bm3d_plan* p);
+// Whether the plan's matching launches use the strip search, and the blocks a
+// matching work-group row owns (tbw without it).
+bool plan_uses_strip(const bm3d_plan* p);
+int plan_match_owned_width(const bm3d_plan* p);
+// dx_spatial_variant of the plan's mixed matching launches.
+int plan_dx_variant(const bm3d_plan* p);
 FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
 VTargetArgs plan_vtarget_args(const bm3d_plan* p);
 

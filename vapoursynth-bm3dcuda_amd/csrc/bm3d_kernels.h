@@ -50,7 +50,16 @@ struct MatchArgs {
     int cost;                // block distance, 0 .. kNumCosts - 1 (the mixed kernel: 0 only)
     int kt;                  // mixed kernel: keys kept per temporal frame (4, 8 or 16)
     int mx_spatial;          // mixed kernel: spatial search on the matrix cores where eligible
+    int strip;               // mixed kernel: spatial search by strips shared between lanes (bs 2 or 4)
+    int obw;                 // blocks owned per work-group row: tbw, or strip_owned_width with strips
 };
+
+// Blocks a work-group row of tbw lanes owns with the strip search: the last
+// 8 / bs - 1 lanes hold only strips of the row's last blocks.
+inline int strip_owned_width(int tbw, int bs) { return tbw - 8 / bs + 1; }
+// The strip search serves block steps 4 (rows of 16 or 32 lanes) and 2 (rows
+// of 16).
+inline bool strip_supported(int tbw, int bs) { return bs == 4 || (bs == 2 && tbw == 16); }
 
 struct FilterArgs {
     const float* src;        // noisy clip, frame 0 plane 0
@@ -199,10 +208,10 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 // variant; lds_bytes covers the largest job.
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, int cost, hipStream_t s);
-// (every job of one launch has the same MatchArgs::kt, and div8 is
-// dx_block_step_divides_8 of each job's block step)
+// (every job of one launch has the same MatchArgs::kt, and `variant` is
+// dx_spatial_variant of each job)
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int kt, bool div8, hipStream_t s);
+                                 size_t lds_bytes, int kt, int variant, hipStream_t s);
 // generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);
@@ -227,9 +236,14 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
 hipError_t set_match_dx_lds_limit(size_t bytes);
 // Default keys kept per temporal frame: 8 (16 for ps_num above 4).
 int dx_temporal_keys(int ps_num);
-// The mixed matcher has one kernel variant for block steps dividing 8 and
-// one for the others.
+// The mixed matcher's kernel variants, by spatial search: per-lane scans
+// (block steps dividing 8, and the others), the matrix-core search, and the
+// strip searches (block step 4 with rows of 16 or 32 lanes, block step 2).
+// Each variant keeps the per-lane scan for the tiles its search does not
+// cover.
+enum { kSpLane = 0, kSpOther, kSpMx, kSpStrip4, kSpStrip4W, kSpStrip2 };
 bool dx_block_step_divides_8(int block_step);
+int dx_spatial_variant(const MatchArgs& a);
 // Grid (ntx, nty, grid_z): grid_z = evaluations (mode 0) or chunks (mode 1);
 // `waves` waves per work-group (1..kMaxFilterWaves).
 constexpr int kMaxFilterWaves = 4;

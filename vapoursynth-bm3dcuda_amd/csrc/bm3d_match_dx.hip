@@ -336,18 +336,15 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
                 }
             }
         }
-        switch (n - base) {
-            case 7: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4)
-                    BM3D_DX_STEP(5) BM3D_DX_STEP(6) break;
-            case 6: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4)
-                    BM3D_DX_STEP(5) break;
-            case 5: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) BM3D_DX_STEP(4) break;
-            case 4: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) BM3D_DX_STEP(3) break;
-            case 3: BM3D_DX_STEP(0) BM3D_DX_STEP(1) BM3D_DX_STEP(2) break;
-            case 2: BM3D_DX_STEP(0) BM3D_DX_STEP(1) break;
-            case 1: BM3D_DX_STEP(0) break;
-            default: break;
-        }
+        // the 0..7 remaining steps, each behind a uniform branch (one copy of
+        // each step's code instead of one per remainder length)
+        if (base < n) BM3D_DX_STEP(0)
+        if (base + 1 < n) BM3D_DX_STEP(1)
+        if (base + 2 < n) BM3D_DX_STEP(2)
+        if (base + 3 < n) BM3D_DX_STEP(3)
+        if (base + 4 < n) BM3D_DX_STEP(4)
+        if (base + 5 < n) BM3D_DX_STEP(5)
+        if (base + 6 < n) BM3D_DX_STEP(6)
 #undef BM3D_DX_STEP
 #undef BM3D_DX_KEY
     }
@@ -364,6 +361,183 @@ __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, cons
         dx_rows<K, kCount, false, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
     else
         dx_rows<K, kCount, true, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// Spatial search with strips shared between lanes (block steps 2 and 4).
+//
+// The reference blocks of a work-group row are bs apart, so the 8 columns of
+// a block are the strips (bs columns x 8 rows) of 8 / bs consecutive lanes.
+// Every lane of a row scans the same displacements (dx, dy) of the window,
+// in the per-lane scan's order, but for its own strip only: half the strip's
+// distance, |c'_s|^2 / 2 + |r'_s|^2 / 2 - <r'_s, c'_s>, is one chain of 4 bs
+// v_dot2 from a sliding sum of per-column squares. The block's half distance
+// is the sum over its strips, taken from the next lanes of the row by DPP
+// (row_shl within rows of 16 lanes; wave_shl for rows of 32 at bs 4). The
+// last 8 / bs - 1 lanes of a row own no block: their strips complete the
+// row's last blocks, so a work-group row owns tbw - 8 / bs + 1 blocks.
+// Keys, scan indices and rescoring are the per-lane scan's; the pre-scores
+// differ from it in FP32 summation order only. Requires every owned block of
+// the tile at x = x0 + lx bs (no block clamped to W - 8).
+
+constexpr int kDppRowShl1 = 0x101, kDppRowShl2 = 0x102, kDppWaveShl1 = 0x130;
+
+// v + (s of the lane kCtrl selects; 0 where it selects none)
+template <int kCtrl>
+__device__ __forceinline__ float add_dpp(float v, float s) {
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, s), kCtrl, 0xF, 0xF,
+                                                                     true));
+}
+
+// Sum of the NS strips starting at this lane.
+template <int NS, bool kWave>
+__device__ __forceinline__ float strip_sum(float v) {
+    if constexpr (NS == 2) {
+        return add_dpp<kWave ? kDppWaveShl1 : kDppRowShl1>(v, v);
+    } else {
+        static_assert(NS == 4 && !kWave, "strips of 2 columns: rows of 16 lanes only");
+        const float t = add_dpp<kDppRowShl1>(v, v);
+        return add_dpp<kDppRowShl2>(t, t);
+    }
+}
+
+// The lane's reference strip (SW columns from xs, read clamped to the plane:
+// a lane without a block may lie beyond it) as negated packed vertical
+// pairs, as ref_pairs; returns |r'_s|^2 / 2.
+template <int SW>
+__device__ __forceinline__ float ref_strip(f16x2 (&rs)[SW][4], const float* __restrict__ plane, int stride, int W,
+                                           int xs, int y, float mu) {
+    float nr = 0.0f;
+#pragma unroll
+    for (int c = 0; c < SW; ++c) {
+        const int col = min(xs + c, W - 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const _Float16 a = (_Float16)(plane[(size_t)(y + 2 * i) * stride + col] - mu);
+            const _Float16 b = (_Float16)(plane[(size_t)(y + 2 * i + 1) * stride + col] - mu);
+            rs[c][i] = f16x2{-a, -b};
+            nr = dot2(rs[c][i], rs[c][i], nr);
+        }
+    }
+    return 0.5f * nr;
+}
+
+// One displacement of the strip scan (step U of a group of 8): the strip's
+// new column (at `addr`), its half distance, the block's sum over strips and
+// the key, as dx_step.
+template <int U, int SW, bool kWave, bool kCount, bool kCheck>
+__device__ __forceinline__ uint32_t strip_step(const f16x2 (&rs)[SW][4], float nrs, f16x2 (&ring)[SW][4],
+                                               float (&csq)[SW], float& N, const uint32_t* __restrict__ addr, int u,
+                                               unsigned vlo, unsigned vspan, uint32_t idx, uint32_t hi_mask,
+                                               unsigned& cnt) {
+    constexpr int kNew = (U + SW - 1) % SW;
+    load_pair_col(ring[kNew], addr);
+    csq[kNew] = col_sq(ring[kNew]);
+    N += csq[kNew];
+    float d = fma_(N, 0.5f, nrs);
+#pragma unroll
+    for (int c = 0; c < SW; ++c) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d = dot2(rs[c][i], ring[(U + c) % SW][i], d);
+    }
+    N -= csq[U % SW];
+    const float approx = fmaxf(strip_sum<8 / SW, kWave>(d), 0.0f);
+    const uint32_t key = (__builtin_bit_cast(uint32_t, approx) & hi_mask) | idx;
+    if constexpr (!kCheck) {
+        if constexpr (kCount) cnt += 1u;
+        return key;
+    } else {
+        const bool ok = (unsigned)u - vlo <= vspan;
+        if constexpr (kCount) cnt += ok ? 1u : 0u;
+        return ok ? key : 0xFFFFFFFFu;
+    }
+}
+
+// Strip scan of the window of radius `range` around the block at (xs, y)
+// (xs: the lane's strip, = its block's x when it owns one).
+template <int K, int SW, bool kWave, bool kCount, bool kCheck>
+__device__ __forceinline__ void strip_rows(const uint32_t* __restrict__ lds, const PairGeom& g,
+                                           const f16x2 (&rs)[SW][4], float nrs, KeyList<K>& kl, int xs, int y,
+                                           int range, int W, int H, unsigned& cnt) {
+    const int n = 2 * range + 1;
+    const int inc8 = (8 / SW) * g.hp2;  // 8 columns on: the same phase (bs = SW divides 8)
+    uint32_t hi_mask;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(hi_mask) : "i"(~kIdxMask));
+    f16x2 ring[SW][4];
+    float csq[SW];
+    for (int dyi = 0; dyi < n; ++dyi) {
+        const int row = y - range + dyi;
+        const int rt = row - g.ty0;
+        const int off = (rt & 1) * g.plane + (rt >> 1);
+        const int tc0 = xs - range - g.tx0;
+        const int q = tc0 / g.bs;
+        int ph = tc0 - q * g.bs;
+        int colbase = (ph * g.wq + q) * g.hp2;
+        float N = 0.0f;
+#pragma unroll
+        for (int s = 0; s + 1 < SW; ++s) {
+            load_pair_col(ring[s], lds + off + colbase);
+            next_pair_col(g, ph, colbase);
+            csq[s] = col_sq(ring[s]);
+            N += csq[s];
+        }
+        const uint32_t* ad[8];  // the new column of steps base .. base + 7
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ad[k] = lds + off + colbase;
+            next_pair_col(g, ph, colbase);
+        }
+        const int col0 = xs - range;
+        const int lo = max(0, -col0), hi = min(n - 1, W - 8 - col0);
+        const bool any = row >= 0 && row <= H - 8 && lo <= hi;
+        const unsigned vlo = any ? (unsigned)lo : 0x40000000u;
+        const unsigned vspan = any ? (unsigned)(hi - lo) : 0u;
+        const uint32_t rowidx = (uint32_t)(dyi * n);
+        int base = 0;
+#define BM3D_ST_KEY(U)                                                                                         \
+    strip_step<U, SW, kWave, kCount, kCheck>(rs, nrs, ring, csq, N, ad[U], base + U, vlo, vspan,                 \
+                                             rowidx + (uint32_t)(base + U), hi_mask, cnt)
+#define BM3D_ST_STEP(U) kl.push(BM3D_ST_KEY(U));
+        for (; base + 8 <= n; base += 8) {
+            uint32_t keys[8];
+            keys[0] = BM3D_ST_KEY(0);
+            keys[1] = BM3D_ST_KEY(1);
+            keys[2] = BM3D_ST_KEY(2);
+            keys[3] = BM3D_ST_KEY(3);
+            keys[4] = BM3D_ST_KEY(4);
+            keys[5] = BM3D_ST_KEY(5);
+            keys[6] = BM3D_ST_KEY(6);
+            keys[7] = BM3D_ST_KEY(7);
+            kl.push8(keys);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) ad[k] += inc8;
+        }
+        if (base < n) BM3D_ST_STEP(0)
+        if (base + 1 < n) BM3D_ST_STEP(1)
+        if (base + 2 < n) BM3D_ST_STEP(2)
+        if (base + 3 < n) BM3D_ST_STEP(3)
+        if (base + 4 < n) BM3D_ST_STEP(4)
+        if (base + 5 < n) BM3D_ST_STEP(5)
+        if (base + 6 < n) BM3D_ST_STEP(6)
+#undef BM3D_ST_STEP
+#undef BM3D_ST_KEY
+    }
+}
+
+// The spatial search of the lane's block by strips: reference strip, then
+// the scan, without the in-plane checks when every owned block's window
+// lies inside the plane.
+template <int K, int SW, bool kWave, bool kCount>
+__device__ __forceinline__ void strip_window(const uint32_t* __restrict__ lds, const PairGeom& g,
+                                             const float* __restrict__ plane, int stride, float mu, KeyList<K>& kl,
+                                             int xs, int y, bool own, int range, int W, int H, unsigned& cnt) {
+    f16x2 rs[SW][4];
+    const float nrs = ref_strip<SW>(rs, plane, stride, W, xs, y, mu);
+    const bool inside = !own || (xs - range >= 0 && xs + range <= W - 8 && y - range >= 0 && y + range <= H - 8);
+    if (__all(inside))
+        strip_rows<K, SW, kWave, kCount, false>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
+    else
+        strip_rows<K, SW, kWave, kCount, true>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,11 +828,14 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
 
 }  // namespace
 
-// One lane per reference block; a work-group covers TBW x TBY blocks (TBW =
-// a.tbw, 16 or 32; TBY = blockDim.x / TBW); grid (ceil(nbx / TBW),
-// ceil(nby / TBY), frames). kKt: keys kept per temporal frame (>= 2 ps_num).
-template <bool kCount, int kKt, bool kDiv8>
+// One lane per reference block; a work-group covers TBW x TBY lanes (TBW =
+// a.tbw, 16 or 32; TBY = blockDim.x / TBW) and owns a.obw x TBY blocks (obw
+// = TBW, or fewer with the strip search: the last lanes of each row then
+// own no block); grid (ceil(nbx / obw), ceil(nby / TBY), frames). kKt: keys
+// kept per temporal frame (>= 2 ps_num).
+template <bool kCount, int kKt, int kSp>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
+    constexpr bool kDiv8 = kSp != kSpOther;
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
     __shared__ uint32_t centre_s[256][8];  // the spatial list, per lane (before it: mx_window's exchange)
@@ -669,9 +846,10 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     const int f = w.z;
     const int n = a.first_frame + f;
     const int r = a.radius;
-    const int bx0 = w.x * a.tbw, by0 = w.y * tby;
+    const int bx0 = w.x * a.obw, by0 = w.y * tby;
     const int bxi = bx0 + lx, byi = by0 + ly;
-    const bool active = bxi < a.nbx && byi < a.nby;
+    const bool own = lx < a.obw;
+    const bool active = own && bxi < a.nbx && byi < a.nby;
     const int x = min(min(bxi, a.nbx - 1) * a.bs, a.W - 8);
     const int y = min(min(byi, a.nby - 1) * a.bs, a.H - 8);
 
@@ -718,9 +896,23 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
         if (fp16) {
             KeyList<kKSpatial> kl;
             kl.init();
-            bool mx = false;
-            if constexpr (kDiv8) mx = mx_eligible(a, bx0) && (tby & 3) == 0;
-            if (mx) {
+            bool mx = false, strip = false;
+            if constexpr (kSp == kSpMx) mx = mx_eligible(a, bx0) && (tby & 3) == 0;
+            // strips: every owned block of the tile on the grid x0 + lx bs
+            if constexpr (kSp >= kSpStrip4) strip = (min(bx0 + a.obw, a.nbx) - 1) * a.bs <= a.W - 8;
+            if (strip) {
+                const int xs = (bx0 + lx) * a.bs;
+                if constexpr (kSp == kSpStrip4)
+                    strip_window<kKSpatial, 4, false, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                                                              a.bm_range, a.W, a.H, cnt);
+                else if constexpr (kSp == kSpStrip4W)
+                    strip_window<kKSpatial, 4, true, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                                                             a.bm_range, a.W, a.H, cnt);
+                else if constexpr (kSp == kSpStrip2)
+                    strip_window<kKSpatial, 2, false, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                                                              a.bm_range, a.W, a.H, cnt);
+            } else if (mx) {
+                if constexpr (kSp == kSpMx) {
                 const int wv = threadIdx.x >> 6;
                 int yg[4];
                 bool row_on[4];
@@ -732,6 +924,7 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
                 }
                 mx_window<kCount>(lds, pg, a, plane_r, mu, bx0 * a.bs, yg, row_on,
                                   &centre_s[64 * wv][0], mx_cn[wv], kl, cnt_wave);
+                }
             } else {
                 f16x2 rp[8][4];
                 const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
@@ -848,66 +1041,72 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     }
 }
 
-template <bool kCount, int kKt, bool kDiv8>
+// kSp: the spatial search variant (kSp* above; dx_spatial_variant).
+template <bool kCount, int kKt, int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx(MatchArgs a) {
-    match_dx_body<kCount, kKt, kDiv8>(a, launch_wg());
+    match_dx_body<kCount, kKt, kSp>(a, launch_wg());
 }
 
-template <int kKt, bool kDiv8>
+template <int kKt, int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, kKt, kDiv8>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, kKt, kSp>(a, job_wg(grids[j], blockIdx.x));
 }
 
 namespace {
-template <int kKt, bool kDiv8>
+template <int kKt, int kSp>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
-    dim3 grid((a.nbx + a.tbw - 1) / a.tbw, (a.nby + tby - 1) / tby, frames);
+    dim3 grid((a.nbx + a.obw - 1) / a.obw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kDiv8>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kDiv8>), grid, block, lds_bytes, s, a);
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kSp>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kSp>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
-template <int kKt, bool kDiv8>
+template <int kKt, int kSp>
 hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                            size_t lds_bytes, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
-    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt, kDiv8>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids,
+    hipLaunchKernelGGL((bm3d_match_dx_multi<kKt, kSp>), dim3(wgs), dim3(threads), lds_bytes, s, jobs, grids,
                        njobs);
     return hipGetLastError();
 }
-template <int kKt, bool kDiv8>
+template <int kKt, int kSp>
 hipError_t set_dx_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kDiv8>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kDiv8>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kDiv8>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kSp>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSp>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kSp>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
-template <bool kDiv8>
+template <int kSp>
 hipError_t launch_dx_keys(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
-    BM3D_INJECT_LAUNCH();
     switch (a.kt) {
-        case 4: return launch_dx<4, kDiv8>(a, tby, lds_bytes, frames, s);
-        case 8: return launch_dx<8, kDiv8>(a, tby, lds_bytes, frames, s);
-        default: return launch_dx<16, kDiv8>(a, tby, lds_bytes, frames, s);
+        case 4: return launch_dx<4, kSp>(a, tby, lds_bytes, frames, s);
+        case 8: return launch_dx<8, kSp>(a, tby, lds_bytes, frames, s);
+        default: return launch_dx<16, kSp>(a, tby, lds_bytes, frames, s);
     }
 }
-template <bool kDiv8>
+template <int kSp>
 hipError_t launch_dx_multi_keys(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                                 size_t lds_bytes, int kt, hipStream_t s) {
-    BM3D_INJECT_LAUNCH();
     switch (kt) {
-        case 4: return launch_dx_multi<4, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
-        case 8: return launch_dx_multi<8, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
-        default: return launch_dx_multi<16, kDiv8>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        case 4: return launch_dx_multi<4, kSp>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        case 8: return launch_dx_multi<8, kSp>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
+        default: return launch_dx_multi<16, kSp>(jobs, grids, njobs, wgs, threads, lds_bytes, s);
     }
+}
+template <int kSp>
+hipError_t set_dx_limits(size_t bytes) {
+    hipError_t e = set_dx_limit<4, kSp>(bytes);
+    if (e == hipSuccess) e = set_dx_limit<8, kSp>(bytes);
+    if (e == hipSuccess) e = set_dx_limit<16, kSp>(bytes);
+    return e;
 }
 }  // namespace
 
@@ -915,23 +1114,41 @@ int dx_temporal_keys(int ps_num) { return ps_num <= 4 ? 8 : 16; }
 
 bool dx_block_step_divides_8(int block_step) { return 8 % block_step == 0; }
 
+int dx_spatial_variant(const MatchArgs& a) {
+    if (!dx_block_step_divides_8(a.bs)) return kSpOther;
+    if (a.strip) return a.bs == 2 ? kSpStrip2 : a.tbw == 32 ? kSpStrip4W : kSpStrip4;
+    return a.mx_spatial ? kSpMx : kSpLane;
+}
+
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
-    return dx_block_step_divides_8(a.bs) ? launch_dx_keys<true>(a, tby, lds_bytes, frames, s)
-                                         : launch_dx_keys<false>(a, tby, lds_bytes, frames, s);
+    switch (dx_spatial_variant(a)) {
+        case kSpOther: return launch_dx_keys<kSpOther>(a, tby, lds_bytes, frames, s);
+        case kSpMx: return launch_dx_keys<kSpMx>(a, tby, lds_bytes, frames, s);
+        case kSpStrip4: return launch_dx_keys<kSpStrip4>(a, tby, lds_bytes, frames, s);
+        case kSpStrip4W: return launch_dx_keys<kSpStrip4W>(a, tby, lds_bytes, frames, s);
+        case kSpStrip2: return launch_dx_keys<kSpStrip2>(a, tby, lds_bytes, frames, s);
+        default: return launch_dx_keys<kSpLane>(a, tby, lds_bytes, frames, s);
+    }
 }
 
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int kt, bool div8, hipStream_t s) {
+                                 size_t lds_bytes, int kt, int variant, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
-    return div8 ? launch_dx_multi_keys<true>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s)
-                : launch_dx_multi_keys<false>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+    switch (variant) {
+        case kSpOther: return launch_dx_multi_keys<kSpOther>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+        case kSpMx: return launch_dx_multi_keys<kSpMx>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+        case kSpStrip4: return launch_dx_multi_keys<kSpStrip4>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+        case kSpStrip4W: return launch_dx_multi_keys<kSpStrip4W>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+        case kSpStrip2: return launch_dx_multi_keys<kSpStrip2>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+        default: return launch_dx_multi_keys<kSpLane>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
+    }
 }
 
 hipError_t set_match_dx_lds_limit(size_t bytes) {
     hipError_t e = hipSuccess;
-    for (hipError_t (*f)(size_t) : {set_dx_limit<4, true>, set_dx_limit<8, true>, set_dx_limit<16, true>,
-                                    set_dx_limit<4, false>, set_dx_limit<8, false>, set_dx_limit<16, false>})
+    for (hipError_t (*f)(size_t) : {set_dx_limits<kSpLane>, set_dx_limits<kSpOther>, set_dx_limits<kSpMx>,
+                                    set_dx_limits<kSpStrip4>, set_dx_limits<kSpStrip4W>, set_dx_limits<kSpStrip2>})
         if (e == hipSuccess) e = f(bytes);
     return e;
 }
