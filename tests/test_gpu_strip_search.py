@@ -1,5 +1,5 @@
 """The mixed matcher's strip search (matching kernel 3, the default at block
-steps 2 and 4 with radius 0).
+steps 2 and 4).
 
 Reference blocks bs apart overlap, so a block's 8 columns are the strips (bs
 columns x 8 rows) of 8 / bs neighbouring lanes: each lane pre-scores its own
@@ -15,7 +15,10 @@ order only, and the same exact rescoring follows, so:
   * with extractor_exp the stacks of both searches are bit-identical wherever
     their lists are.
 Rows of 16 lanes (bs 2 and 4) and of 32 (bs 4) are covered; bs 2 with rows of
-32, and other block steps, keep the per-lane scan.
+32, and other block steps, keep the per-lane scan. At radius > 0 a plan runs
+the strip search as two launches (spatial by strips, then temporal with one
+lane per block); bm3d_execute_noalloc runs it in one launch, and
+test_zz_no_alloc.py holds the two forms' stacks bit-identical.
 """
 import numpy as np
 import pytest
@@ -61,8 +64,6 @@ def test_strip_lists_counts_and_stacks(case, final):
     ref = torch.from_numpy(np.clip(clean + np.float32(0.002), 0, 1).astype(np.float32)).cuda() if final else None
     kw = dict(block_step=bs, bm_range=R, radius=r, ps_num=pn, ps_range=pr, final=final, extractor_exp=10)
     ps = core.Plan(W, H, 15.0, precision="mixed", **kw)
-    assert ps.uses_strip_search() == (r == 0)
-    ps.select_kernels("mixed-strip", "mixed")
     pl = core.Plan(W, H, 15.0, precision="mixed", **kw)
     pl.select_kernels("mixed", "mixed")
     p32 = core.Plan(W, H, 15.0, **kw)
@@ -87,8 +88,8 @@ def test_strip_lists_counts_and_stacks(case, final):
 
 def test_strip_search_selection():
     """Strips serve block steps 4 (rows of 16 or 32 lanes) and 2 (rows of
-    16), by default at radius 0; elsewhere a plan keeps the per-lane search,
-    and selecting the strip search there is not an error."""
+    16), by default; elsewhere a plan keeps the per-lane search, and selecting
+    the strip search there is not an error."""
     core = _core()
     W, H = 128, 96
     for bs, mw, want in [(4, 16, True), (4, 32, True), (2, 16, True), (2, 32, False), (8, 16, False),
@@ -97,7 +98,9 @@ def test_strip_search_selection():
         p.set_tuning(match_width=mw, match_rows=8)
         assert p.uses_mixed_kernel()
         assert p.uses_strip_search() == want, (bs, mw)
-        assert not core.Plan(W, H, 10.0, block_step=bs, bm_range=6, radius=1, precision="mixed").uses_strip_search()
+        p1 = core.Plan(W, H, 10.0, block_step=bs, bm_range=6, radius=1, precision="mixed")
+        p1.set_tuning(match_width=mw, match_rows=8)
+        assert p1.uses_strip_search() == want
         p.select_kernels("mixed", "mixed")
         assert not p.uses_strip_search()
         p.select_kernels("mixed-strip", "mixed")

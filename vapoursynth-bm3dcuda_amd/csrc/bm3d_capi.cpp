@@ -411,16 +411,68 @@ bm3d::MatchArgs match_args(const bm3d_plan* p, uint32_t* groups, unsigned long l
     m.kt = bm3d::plan_temporal_keys(p);
     m.mx_spatial = p->mx_spatial ? 1 : 0;
     m.strip = bm3d::plan_uses_strip(p) ? 1 : 0;
-    m.obw = bm3d::plan_match_owned_width(p);
+    m.obw = m.strip ? bm3d::strip_owned_width(p->tbw, d.block_step) : p->tbw;
     return m;
 }
 
-hipError_t launch_matching(const bm3d_plan* p, uint32_t* groups, unsigned long long* ncand,
-                           const bm3d_clip* src, const bm3d_clip* ref, int first_frame, int count,
-                           hipStream_t stream) {
+// Frames per pair of launches of the two-launch strip search: their
+// hand-over lists (64 bytes per block) stay within 256 MB.
+int split_frames(const bm3d_plan* p, int count) {
+    const long long per_frame = (long long)p->nblocks * 16 * sizeof(uint32_t);
+    return (int)std::max(1LL, std::min<long long>(count, (256LL << 20) / per_frame));
+}
+
+// The hand-over lists of `frames` frames of the two-launch strip search,
+// grown if needed; null (*e unchanged) when the buffer is too small and may
+// not move (a capturing stream or a plan held by a graph): the search then
+// runs in one launch, with the same results.
+uint32_t* split_lists(bm3d_plan* p, int frames, hipStream_t stream, hipError_t* e, bool may_alloc) {
+    const size_t need = (size_t)frames * p->nblocks * 16;
+    if (need <= p->slist_cap) return p->slist;
+    if (!may_alloc) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    *e = hipStreamIsCapturing(stream, &cs);
+    if (*e != hipSuccess || cs != hipStreamCaptureStatusNone || p->pinned) return nullptr;
+    if (p->slist) (void)hipFree(p->slist);
+    p->slist = nullptr;
+    p->slist_cap = 0;
+    *e = bm3d::malloc_device(&p->slist, need * sizeof(uint32_t));
+    if (*e != hipSuccess) return nullptr;
+    p->slist_cap = need;
+    return p->slist;
+}
+
+hipError_t launch_matching(bm3d_plan* p, uint32_t* groups, unsigned long long* ncand, const bm3d_clip* src,
+                           const bm3d_clip* ref, int first_frame, int count, hipStream_t stream,
+                           bool may_alloc = true) {
     const bm3d::MatchArgs m = match_args(p, groups, ncand, src, ref, first_frame);
-    if (p->use_dx) return bm3d::launch_match_dx(m, p->tby, p->dx_lds_bytes, count, stream);
-    return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
+    if (!p->use_dx) return bm3d::launch_match(m, p->tby, p->lds_bytes, count, stream);
+    if (m.strip && p->d.radius > 0 && count > 0 && may_alloc) {
+        // the spatial search by strips, then the temporal search with one
+        // lane per block (the strips' last lanes own no block and would idle
+        // through it)
+        const int chunk = split_frames(p, count);
+        hipError_t e = hipSuccess;
+        if (uint32_t* sl = split_lists(p, chunk, stream, &e, may_alloc)) {
+            for (int c0 = 0; c0 < count && e == hipSuccess; c0 += chunk) {
+                const int nf = std::min(chunk, count - c0);
+                bm3d::MatchArgs ms = m;
+                ms.first_frame = first_frame + c0;
+                ms.slist = sl;
+                ms.phase = bm3d::kPhSpatial;
+                bm3d::MatchArgs mt = ms;
+                mt.phase = bm3d::kPhTemporal;
+                mt.strip = 0;
+                mt.obw = mt.tbw;
+                mt.groups = groups ? groups + (size_t)c0 * p->nblocks * 8 : nullptr;
+                e = bm3d::launch_match_dx(ms, p->tby, p->dx_lds_bytes, nf, stream);
+                if (e == hipSuccess) e = bm3d::launch_match_dx(mt, p->tby, p->dx_lds_bytes, nf, stream);
+            }
+            return e;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return bm3d::launch_match_dx(m, p->tby, p->dx_lds_bytes, count, stream);
 }
 
 // Range pointer of the fixed-point unit (null: the unit is fixed).
@@ -515,9 +567,10 @@ struct SlotBufs {
     int* wrap;
     int* regions;
     bool wrap_always_zero;  // zero the wrap counts every batch (else only once the plan counted a wrap)
+    bool may_alloc;         // matching may allocate its own buffers (not in bm3d_execute_noalloc)
 };
 
-SlotBufs plan_slot_bufs(const bm3d_plan* p) { return {p->groups, p->corner, p->wrap, p->scratch, false}; }
+SlotBufs plan_slot_bufs(const bm3d_plan* p) { return {p->groups, p->corner, p->wrap, p->scratch, false, true}; }
 
 hipError_t evaluate_slots(bm3d_plan* p, const SlotBufs& b, const bm3d_clip* src, const bm3d_clip* ref, int n0,
                           int nf, float* res, long long res_frame_stride, long long res_plane_stride,
@@ -530,7 +583,7 @@ hipError_t evaluate_slots(bm3d_plan* p, const SlotBufs& b, const bm3d_clip* src,
     if (e != hipSuccess) return e;
     {
         ProfScope ps(0, stream);
-        e = launch_matching(p, b.groups, nullptr, src, ref, n0, nf, stream);
+        e = launch_matching(p, b.groups, nullptr, src, ref, n0, nf, stream, b.may_alloc);
     }
     if (e != hipSuccess) return e;
     bm3d::FilterArgs fa = filter_args(p, src, ref);
@@ -669,19 +722,26 @@ int plan_temporal_keys(const bm3d_plan* p) {
 bool plan_uses_strip(const bm3d_plan* p) {
     return p->use_dx && p->strip && !p->mx_spatial && p->lds_bytes && strip_supported(p->tbw, p->d.block_step);
 }
+// Batches run the whole search in one launch over all their jobs (at radius
+// > 0 the strip search's last lanes then idle through the temporal search;
+// the lists are the two-launch form's).
+bool plan_batch_strip(const bm3d_plan* p) { return plan_uses_strip(p); }
 int plan_match_owned_width(const bm3d_plan* p) {
-    return plan_uses_strip(p) ? strip_owned_width(p->tbw, p->d.block_step) : p->tbw;
+    return plan_batch_strip(p) ? strip_owned_width(p->tbw, p->d.block_step) : p->tbw;
 }
 int plan_dx_variant(const bm3d_plan* p) {
     MatchArgs m{};
     m.bs = p->d.block_step;
     m.tbw = p->tbw;
-    m.strip = plan_uses_strip(p) ? 1 : 0;
+    m.strip = plan_batch_strip(p) ? 1 : 0;
     m.mx_spatial = p->mx_spatial ? 1 : 0;
     return dx_spatial_variant(m);
 }
 MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
-    return match_args(p, nullptr, nullptr, src, ref, 0);
+    MatchArgs m = match_args(p, nullptr, nullptr, src, ref, 0);
+    m.strip = plan_batch_strip(p) ? 1 : 0;
+    m.obw = plan_match_owned_width(p);
+    return m;
 }
 FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref) {
     return filter_args(p, src, ref);
@@ -789,11 +849,8 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
         return rc;
     }
     p->use_dx = p->dx_ok;
-    // the strip search shortens the spatial search by about a fifth, but its
-    // last lanes per row own no block and idle through a temporal search: at
-    // radius 1 on the 1080p benchmark clip it was 5% slower than the per-lane
-    // search, at radius 0 17% faster
-    p->strip = d.radius == 0;
+    // the strip search: about a fifth fewer instructions in the spatial search
+    p->strip = true;
     p->vbound = bm3d::contribution_bound(d.transform_2d, d.transform_1d, p->d.final_ != 0);
     e = bm3d::malloc_device(&p->range, 256);  // the range, then the wrap flag
     if (e == hipSuccess) e = hipMemset(p->range, 0, 256);
@@ -822,6 +879,7 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
 void bm3d_plan_destroy(bm3d_plan* p) {
     if (!p) return;
     if (p->groups) (void)hipFree(p->groups);
+    if (p->slist) (void)hipFree(p->slist);
     if (p->corner) (void)hipFree(p->corner);
     if (p->wrap) (void)hipFree(p->wrap);
     if (p->part) (void)hipFree(p->part);
@@ -864,6 +922,7 @@ int bm3d_execute_noalloc(bm3d_plan* p, float* d_res, long long res_frame_stride,
     c += corner_b * batch;
     b.regions = reinterpret_cast<int*>(c);
     b.wrap_always_zero = true;  // the array keeps other calls' counts
+    b.may_alloc = false;        // matching in one launch (same results)
     return run_slots(p, b, batch, d_res, res_frame_stride, res_plane_stride, src, ref, first_frame, count,
                      reinterpret_cast<hipStream_t>(stream_));
 }

@@ -68,8 +68,13 @@ struct bm3d_plan {
     bool use_dx = false, dx_ok = false;
     bool mx_spatial = false;  // the mixed matcher's spatial search on the matrix cores (where eligible; opt-in)
     // the mixed matcher's spatial search by strips shared between lanes where
-    // the block step and tile allow it (strip_supported); the default at radius 0
+    // the block step and tile allow it (strip_supported)
     bool strip = false;
+    // at radius > 0 the strip search runs as two launches (spatial by strips,
+    // then temporal, one lane per block) handing the spatial best 8 over in
+    // slist (16 words per block and frame, slist_cap words)
+    uint32_t* slist = nullptr;
+    size_t slist_cap = 0;
     int tile_hp2 = 0;
     size_t dx_lds_bytes = 0;
     // mixed-precision filter kernel (precision MIXED): per-lane transform operands
@@ -94,11 +99,14 @@ MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_c
 // Keys the mixed matcher keeps per temporal frame: desc.mixed_temporal_keys,
 // or dx_temporal_keys(ps_num) when that is 0.
 int plan_temporal_keys(const bm3d_plan* p);
-// Whether the plan's matching launches use the strip search, and the blocks a
-// matching work-group row owns (tbw without it).
+// Whether the plan's matching uses the strip search (at radius > 0 as two
+// launches, spatial then temporal); whether a batch's one-launch matching
+// does (as the plan's, with the same lists); the blocks a batch's matching work-group row owns
+// (tbw without strips); the batch's dx_spatial_variant of the plan.
+// plan_match_args describes the batch's launches.
 bool plan_uses_strip(const bm3d_plan* p);
+bool plan_batch_strip(const bm3d_plan* p);
 int plan_match_owned_width(const bm3d_plan* p);
-// dx_spatial_variant of the plan's mixed matching launches.
 int plan_dx_variant(const bm3d_plan* p);
 FilterArgs plan_filter_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_clip* ref);
 VTargetArgs plan_vtarget_args(const bm3d_plan* p);

@@ -52,7 +52,14 @@ struct MatchArgs {
     int mx_spatial;          // mixed kernel: spatial search on the matrix cores where eligible
     int strip;               // mixed kernel: spatial search by strips shared between lanes (bs 2 or 4)
     int obw;                 // blocks owned per work-group row: tbw, or strip_owned_width with strips
+    // mixed kernel at radius > 0: the whole search in one launch (0), or its
+    // spatial part (1, strips only) then its temporal part (2) in two launches
+    // handing the spatial best 8 over in slist, [frames][nblocks][16]
+    // (8 positions, then the bits of their 8 distances)
+    int phase;
+    uint32_t* slist;
 };
+enum { kPhBoth = 0, kPhSpatial, kPhTemporal };
 
 // Blocks a work-group row of tbw lanes owns with the strip search: the last
 // 8 / bs - 1 lanes hold only strips of the row's last blocks.

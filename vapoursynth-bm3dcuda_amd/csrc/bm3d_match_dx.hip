@@ -833,7 +833,10 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
 // = TBW, or fewer with the strip search: the last lanes of each row then
 // own no block); grid (ceil(nbx / obw), ceil(nby / TBY), frames). kKt: keys
 // kept per temporal frame (>= 2 ps_num).
-template <bool kCount, int kKt, int kSp>
+// kPh: the whole search (kPhBoth), or at radius > 0 its spatial part alone
+// (kPhSpatial: the best 8 go to a.slist) or its temporal part alone
+// (kPhTemporal: starts from a.slist), as two launches of one evaluation.
+template <bool kCount, int kKt, int kSp, int kPh>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     constexpr bool kDiv8 = kSp != kSpOther;
     extern __shared__ __align__(16) float lds_f[];
@@ -890,7 +893,25 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
 
     unsigned cnt = 0, cnt_wave = 0;  // cnt_wave: counted for other lanes' blocks too
     Best8 best;
-    {
+    // the block's spatial list between the two launches (active lanes only)
+    auto slist_at = [&]() -> uint32_t* {
+        return a.slist + ((size_t)f * a.nblocks + (size_t)byi * a.nbx + bxi) * 16;
+    };
+    if constexpr (kPh == kPhTemporal) {
+        // the spatial best 8 of the first launch: positions, then distances
+        best.init(kFltMax, pack_pos(x, y, r));
+        if (active) {
+            const uint4* in = reinterpret_cast<const uint4*>(slist_at());
+            const uint4 p0 = in[0], p1 = in[1], e0 = in[2], e1 = in[3];
+            const uint32_t pv[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+            const uint32_t ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                best.p[i] = pv[i];
+                best.e[i] = __builtin_bit_cast(float, ev[i]);
+            }
+        }
+    } else {
         float mu;
         const bool fp16 = stage(plane_r, mu);
         if (fp16) {
@@ -949,6 +970,24 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
         } else {
             best.init(kFltMax, pack_pos(x, y, r));
             exact_window<kCount>(plane_r, x, y, plane_r, a.stride, x, y, a.bm_range, r, a.W, a.H, best, cnt);
+        }
+    }
+    if constexpr (kPh == kPhSpatial) {
+        if (r > 0) {  // uniform: the temporal launch continues from slist
+            if constexpr (kCount) {
+                const unsigned total = (active ? cnt : 0u) + cnt_wave;
+                if (total) atomicAdd(a.ncand, (unsigned long long)total);
+            }
+            if (active) {
+                uint4* out = reinterpret_cast<uint4*>(slist_at());
+                out[0] = make_uint4(best.p[0], best.p[1], best.p[2], best.p[3]);
+                out[1] = make_uint4(best.p[4], best.p[5], best.p[6], best.p[7]);
+                out[2] = make_uint4(__builtin_bit_cast(uint32_t, best.e[0]), __builtin_bit_cast(uint32_t, best.e[1]),
+                                    __builtin_bit_cast(uint32_t, best.e[2]), __builtin_bit_cast(uint32_t, best.e[3]));
+                out[3] = make_uint4(__builtin_bit_cast(uint32_t, best.e[4]), __builtin_bit_cast(uint32_t, best.e[5]),
+                                    __builtin_bit_cast(uint32_t, best.e[6]), __builtin_bit_cast(uint32_t, best.e[7]));
+            }
+            return;
         }
     }
 
@@ -1041,10 +1080,11 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     }
 }
 
-// kSp: the spatial search variant (kSp* above; dx_spatial_variant).
-template <bool kCount, int kKt, int kSp>
+// kSp: the spatial search variant (kSp* above; dx_spatial_variant); kPh:
+// the part of the search (match_dx_body).
+template <bool kCount, int kKt, int kSp, int kPh>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx(MatchArgs a) {
-    match_dx_body<kCount, kKt, kSp>(a, launch_wg());
+    match_dx_body<kCount, kKt, kSp, kPh>(a, launch_wg());
 }
 
 template <int kKt, int kSp>
@@ -1052,17 +1092,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, kKt, kSp>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, kKt, kSp, kPhBoth>(a, job_wg(grids[j], blockIdx.x));
 }
 
 namespace {
-template <int kKt, int kSp>
+template <int kKt, int kSp, int kPh = kPhBoth>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
     dim3 grid((a.nbx + a.obw - 1) / a.obw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kSp>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kSp>), grid, block, lds_bytes, s, a);
+    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
+    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
     return hipGetLastError();
 }
 template <int kKt, int kSp>
@@ -1075,9 +1115,13 @@ hipError_t launch_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njob
 }
 template <int kKt, int kSp>
 hipError_t set_dx_limit(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kSp>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSp>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kSp>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kSp, kPhBoth>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSp, kPhBoth>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kSp>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kSpLane, kPhTemporal>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSpLane, kPhTemporal>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<false, 8, kSp, kPhSpatial>),
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, 8, kSp, kPhSpatial>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
@@ -1122,6 +1166,20 @@ int dx_spatial_variant(const MatchArgs& a) {
 
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
     BM3D_INJECT_LAUNCH();
+    if (a.phase == kPhTemporal) {
+        switch (a.kt) {
+            case 4: return launch_dx<4, kSpLane, kPhTemporal>(a, tby, lds_bytes, frames, s);
+            case 8: return launch_dx<8, kSpLane, kPhTemporal>(a, tby, lds_bytes, frames, s);
+            default: return launch_dx<16, kSpLane, kPhTemporal>(a, tby, lds_bytes, frames, s);
+        }
+    }
+    if (a.phase == kPhSpatial) {  // the strip searches (the temporal keys do not matter)
+        switch (dx_spatial_variant(a)) {
+            case kSpStrip4W: return launch_dx<8, kSpStrip4W, kPhSpatial>(a, tby, lds_bytes, frames, s);
+            case kSpStrip2: return launch_dx<8, kSpStrip2, kPhSpatial>(a, tby, lds_bytes, frames, s);
+            default: return launch_dx<8, kSpStrip4, kPhSpatial>(a, tby, lds_bytes, frames, s);
+        }
+    }
     switch (dx_spatial_variant(a)) {
         case kSpOther: return launch_dx_keys<kSpOther>(a, tby, lds_bytes, frames, s);
         case kSpMx: return launch_dx_keys<kSpMx>(a, tby, lds_bytes, frames, s);
