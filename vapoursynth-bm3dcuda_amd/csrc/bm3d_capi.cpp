@@ -143,8 +143,10 @@ bm3d_tuning default_tuning(const bm3d_plan* p) {
     t.filter_tile_x = 8;
     t.filter_tile_y = 8;
     // the autotuner's choice for a 30-frame 1080p clip on both paths: four
-    // evaluation runs per launch give the filter 4x the work-groups
-    t.filter_waves = 4;
+    // evaluation runs per launch give the filter 4x the work-groups; the
+    // mixed filter's copy of its transform operands takes 17 KB of LDS per
+    // work-group, so it runs 8 waves per work-group for 4 waves per SIMD
+    t.filter_waves = p->d.precision == BM3D_PRECISION_MIXED ? 8 : 4;
     t.batch_frames = kDefaultBatchFrames;
     t.chunks = 4;
     return t;
@@ -289,7 +291,10 @@ bool filter_geometry(bm3d_plan* p, int tbx, int tby, int rm) {
     const int rw = (tbx - 1) * bs + 8 + 2 * rm;
     const int rh = (tby - 1) * bs + 8 + 2 * rm;
     const size_t bytes = (size_t)p->tw * rw * rh * sizeof(int);
-    if (bytes + 1024 > (size_t)p->max_lds) return false;  // + the kernels' static slot table
+    // + the kernels' static slot table, and the mixed filter's copy of its
+    // transform operands
+    const size_t fixed = 1024 + (p->d.precision == BM3D_PRECISION_MIXED ? bm3d::kMxOpVectors * 64 * 16 : 0);
+    if (bytes + fixed > (size_t)p->max_lds) return false;
     p->ftbx = tbx; p->ftby = tby; p->frm = rm; p->frw = rw; p->frh = rh;
     p->filter_lds_bytes = bytes;
     p->fntx = (p->nbx + tbx - 1) / tbx;
@@ -324,7 +329,9 @@ int apply_tuning(bm3d_plan* p, const bm3d_tuning& t) {
     hipError_t e = raise_lds_limit(0, p->lds_bytes);
     if (e == hipSuccess && p->dx_ok) e = raise_lds_limit(1, p->dx_lds_bytes);
     if (e == hipSuccess) e = raise_lds_limit(2, p->filter_lds_bytes);
-    if (e == hipSuccess) e = raise_lds_limit(3, p->filter_lds_bytes);
+    // (only mixed-precision plans can run the mixed filter, whose static LDS
+    // leaves less room for the regions)
+    if (e == hipSuccess && p->d.precision == BM3D_PRECISION_MIXED) e = raise_lds_limit(3, p->filter_lds_bytes);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
     p->tune = t;
     p->tune.match_rows = p->tby;

@@ -67,21 +67,34 @@ __device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
-// Per-lane transform operands (kMxOpVectors uint4 per lane, lane-interleaved).
+// Per-lane transform operands (kMxOpVectors uint4 per lane, lane-interleaved),
+// read from a work-group copy in LDS where each product needs them: held in
+// registers for the whole kernel they took 68 VGPRs, and the filter, bound
+// by memory latency, needs the occupancy more than the LDS bandwidth.
 struct MxOps {
-    h8 kf[4][2];  // K^T as the B operand of Y^T = X^T K^T: [tile t][k-step s]
-    h8 ki[4][2];  // K^T as the A operand of X' = K^T W: [tile T][k-step s]
-    h4 dA;        // Dbd as the A operand (forward along members)
-    h4 dB;        // Dbd as the B operand (inverse along members)
-
-    __device__ __forceinline__ void load(const uint4* __restrict__ ops, int lane) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) kf[v >> 1][v & 1] = __builtin_bit_cast(h8, ops[v * 64 + lane]);
-#pragma unroll
-        for (int v = 0; v < 8; ++v) ki[v >> 1][v & 1] = __builtin_bit_cast(h8, ops[(8 + v) * 64 + lane]);
-        const uint4 d = ops[16 * 64 + lane];
-        dA = __builtin_bit_cast(h4, uint2{d.x, d.y});
-        dB = __builtin_bit_cast(h4, uint2{d.z, d.w});
+    const uint4* s;  // LDS copy (this lane's entries: s[v * 64])
+    __device__ __forceinline__ uint4 at(int i) const { return s[i]; }
+    // The copy as seen from one wave step: an opaque zero offset keeps the
+    // compiler from hoisting the reads out of the group loop into registers.
+    __device__ __forceinline__ MxOps step() const {
+        int z;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+        return MxOps{s + z};
+    }
+    // K^T as the B operand of Y^T = X^T K^T: [tile t][k-step k]
+    __device__ __forceinline__ h8 kf(int t, int k) const { return __builtin_bit_cast(h8, at((2 * t + k) * 64)); }
+    // K^T as the A operand of X' = K^T W: [tile T][k-step k]
+    __device__ __forceinline__ h8 ki(int t, int k) const {
+        return __builtin_bit_cast(h8, at((8 + 2 * t + k) * 64));
+    }
+    // Dbd as the A operand (forward along members) and the B operand (inverse)
+    __device__ __forceinline__ h4 dA() const {
+        const uint4 d = at(16 * 64);
+        return __builtin_bit_cast(h4, uint2{d.x, d.y});
+    }
+    __device__ __forceinline__ h4 dB() const {
+        const uint4 d = at(16 * 64);
+        return __builtin_bit_cast(h4, uint2{d.z, d.w});
     }
 };
 
@@ -147,11 +160,12 @@ __device__ __forceinline__ void forward_mx_rows(f4 (&z)[4], float& mu, float& s,
     const h8 a0 = cvt8(v[0], v[1]);
     const h8 a1 = cvt8(v[2], v[3]);
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const h4 dA = o.dA();
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        f4 y = mfma32(a0, o.kf[t][0], zero);
-        y = mfma32(a1, o.kf[t][1], y);
-        z[t] = mfma16(o.dA, cvt4(y), zero);
+        f4 y = mfma32(a0, o.kf(t, 0), zero);
+        y = mfma32(a1, o.kf(t, 1), y);
+        z[t] = mfma16(dA, cvt4(y), zero);
     }
 }
 
@@ -161,14 +175,15 @@ __device__ __forceinline__ void forward_mx_rows(f4 (&z)[4], float& mu, float& s,
 __device__ __forceinline__ void inverse_mx(f4 (&x)[4], const MxOps& o, const f4 (&z)[4]) {
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     h4 w[4];
+    const h4 dB = o.dB();
 #pragma unroll
-    for (int t = 0; t < 4; ++t) w[t] = cvt4(mfma16(cvt4(z[t]), o.dB, zero));
+    for (int t = 0; t < 4; ++t) w[t] = cvt4(mfma16(cvt4(z[t]), dB, zero));
     const h8 b0 = h8{w[0][0], w[0][1], w[0][2], w[0][3], w[1][0], w[1][1], w[1][2], w[1][3]};
     const h8 b1 = h8{w[2][0], w[2][1], w[2][2], w[2][3], w[3][0], w[3][1], w[3][2], w[3][3]};
 #pragma unroll
     for (int T = 0; T < 4; ++T) {
-        f4 v = mfma32(o.ki[T][0], b0, zero);
-        x[T] = mfma32(o.ki[T][1], b1, v);
+        f4 v = mfma32(o.ki(T, 0), b0, zero);
+        x[T] = mfma32(o.ki(T, 1), b1, v);
     }
 }
 
@@ -188,6 +203,7 @@ template <bool kFinal, bool kChroma, bool kWide>
 __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     extern __shared__ float lds[];
     __shared__ SlotTable st;
+    __shared__ uint4 ops_s[kMxOpVectors * 64];
     int* acc = reinterpret_cast<int*>(lds);
     const int nwaves = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
@@ -207,8 +223,10 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     const bool wide = (1.25f * a.vbound * *a.range + E) * fx_scale >= 0x1p30f;
     if (wide != kWide) return;  // uniform over the work-group
 
-    MxOps o;
-    o.load(reinterpret_cast<const uint4*>(a.mx_ops), lane);
+    // (read after filter_walk's first barrier)
+    for (int i = threadIdx.x; i < kMxOpVectors * 64; i += blockDim.x)
+        ops_s[i] = reinterpret_cast<const uint4*>(a.mx_ops)[i];
+    const MxOps ops{ops_s + lane};
 
     filter_walk(a, wg, acc, st, kChroma ? 3 : 1, kChroma, [&](int p, int n, int gf) {
         const float sigma = a.sigma[p];
@@ -240,6 +258,7 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
             const int py = min(pos_y(pk), a.H - 8);
             const int fi = clampi(n - a.radius + z, 0, a.num_frames - 1);
             const float* mp = src + (size_t)fi * a.frame_stride + (size_t)py * a.stride + px;
+            const MxOps o = ops.step();
 
             f4 zc[4];
             float w0, w1, mu, sc;
@@ -373,13 +392,15 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
     });
 }
 
+// 4 waves per SIMD: 128 VGPRs (the transform operands are read from LDS)
 template <bool kFinal, bool kChroma, bool kWide>
-__global__ __launch_bounds__(512) void bm3d_filter_mx(FilterArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_filter_mx(FilterArgs a) {
     filter_mx_body<kFinal, kChroma, kWide>(a, launch_wg());
 }
 
 template <bool kFinal, bool kChroma, bool kWide>
-__global__ __launch_bounds__(512) void bm3d_filter_mx_multi(const FilterArgs* __restrict__ jobs,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_filter_mx_multi(
+    const FilterArgs* __restrict__ jobs,
                                                             const JobGrid* __restrict__ grids, int njobs) {
     const int j = find_job(grids, njobs, blockIdx.x);
     const FilterArgs& a = jobs[j];
