@@ -108,3 +108,45 @@ def test_push8_keeps_the_k_smallest(K, spread):
 def test_kernel_key_count_is_mergeable():
     m = re.search(r"constexpr int kKSpatial = (\d+);", _source())
     assert m and 8 <= int(m.group(1)) <= 16
+
+
+def _insert(k, x):
+    """KeyList::insert: k[0] = min(k[0], x), k[i] = the median of
+    (k[i-1], x, k[i]) for the old k (the clamp of x to [k[i-1], k[i]])."""
+    out = [min(k[0], x)]
+    for i in range(1, len(k)):
+        out.append(sorted((k[i - 1], x, k[i]))[1])
+    return out
+
+
+def test_insert_reads_median_of_three_in_source():
+    """The kernel's insertion computes max(min(prev, key), min(max(prev,
+    key), cur)), the median of three (the form the compiler emits as one
+    v_med3_u32), from the list's old entries."""
+    src = _source()
+    body = src[src.index("void insert(uint32_t key)"):]
+    body = body[:body.index("\n    }\n")]
+    assert "k[0] = min(k[0], key)" in body
+    assert "max(min(prev, key), min(max(prev, key), cur))" in body
+    assert re.search(r"const uint32_t cur = k\[i\];", body) and "prev = cur;" in body
+
+
+@pytest.mark.parametrize("K", [4, 8, 10, 16])
+def test_insert_keeps_the_k_smallest(K):
+    """One key at a time, the list is sorted(keys so far)[:K] (all ones
+    marking empty entries), for random, tied and descending keys."""
+    rng = random.Random(K)
+    empty = 0xFFFFFFFF
+    for trial in range(200):
+        n = rng.randrange(1, 60)
+        if trial % 3 == 0:
+            keys = [rng.randrange(8) for _ in range(n)]
+        elif trial % 3 == 1:
+            keys = sorted((rng.randrange(1 << 32) for _ in range(n)), reverse=True)
+        else:
+            keys = [rng.randrange(1 << 32) for _ in range(n)]
+        k = [empty] * K
+        for j, x in enumerate(keys):
+            k = _insert(k, x)
+            want = (sorted(keys[:j + 1]) + [empty] * K)[:K]
+            assert k == want

@@ -79,6 +79,19 @@ struct KeyList {
 #pragma unroll
         for (int i = 0; i < K; ++i) k[i] = 0xFFFFFFFFu;
     }
+    // Sorted insertion: k[0] = min(k[0], key), k[i] = med3(k[i-1], key, k[i])
+    // (the median of three is the clamp of key to [k[i-1], k[i]], which the
+    // compiler emits as one v_med3_u32): K independent operations per key.
+    __device__ __forceinline__ void insert(uint32_t key) {
+        uint32_t prev = k[0];
+        k[0] = min(k[0], key);
+#pragma unroll
+        for (int i = 1; i < K; ++i) {
+            const uint32_t cur = k[i];
+            k[i] = max(min(prev, key), min(max(prev, key), cur));
+            prev = cur;
+        }
+    }
     // (each entry is updated in place, so the list keeps its registers)
     __device__ __forceinline__ void push(uint32_t key) {
 #pragma unroll
@@ -262,7 +275,7 @@ __device__ __forceinline__ uint32_t dx_step(const f16x2 (&rp)[8][4], float nr, f
 // Approximate scan of the window of radius `range` around (cx, cy) into kl;
 // scan index of (dyi, dxi) is idx_base + dyi * n + dxi. kCheck false: every
 // candidate of the window is in-plane.
-template <int K, bool kCount, bool kCheck, bool kDiv8>
+template <int K, bool kCount, bool kCheck, bool kDiv8, bool kIns>
 __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const PairGeom& g,
                                           const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
                                           int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
@@ -313,18 +326,28 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
 #define BM3D_DX_KEY(U)                                                                               \
     dx_step<U, kCount, kCheck>(rp, nr, ring, csq, N, ad[U], base + U, vlo, vspan, rowidx + (uint32_t)(base + U), \
                                hi_mask, cnt)
-#define BM3D_DX_STEP(U) kl.push(BM3D_DX_KEY(U));
+        // kIns: groups of 8 keys merged at once above 8 kept keys (fewer
+        // operations per key than sorted insertion), inserted one by one up
+        // to 8 (as cheap, with fewer live registers), single steps inserted;
+        // else every group merged and single steps pushed through the cascade
+        // (the form the one-launch kernel allocates its registers best for)
+#define BM3D_DX_STEP(U)                     \
+    if constexpr (kIns) kl.insert(BM3D_DX_KEY(U)); \
+    else kl.push(BM3D_DX_KEY(U));
+#define BM3D_DX_GSTEP(U)                                     \
+    if constexpr (K > 8 || !kIns) keys[U] = BM3D_DX_KEY(U);  \
+    else kl.insert(BM3D_DX_KEY(U));
         for (; base + 8 <= n; base += 8) {
             uint32_t keys[8];
-            keys[0] = BM3D_DX_KEY(0);
-            keys[1] = BM3D_DX_KEY(1);
-            keys[2] = BM3D_DX_KEY(2);
-            keys[3] = BM3D_DX_KEY(3);
-            keys[4] = BM3D_DX_KEY(4);
-            keys[5] = BM3D_DX_KEY(5);
-            keys[6] = BM3D_DX_KEY(6);
-            keys[7] = BM3D_DX_KEY(7);
-            kl.push8(keys);
+            BM3D_DX_GSTEP(0)
+            BM3D_DX_GSTEP(1)
+            BM3D_DX_GSTEP(2)
+            BM3D_DX_GSTEP(3)
+            BM3D_DX_GSTEP(4)
+            BM3D_DX_GSTEP(5)
+            BM3D_DX_GSTEP(6)
+            BM3D_DX_GSTEP(7)
+            if constexpr (K > 8 || !kIns) kl.push8(keys);
             if constexpr (kDiv8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) ad[k] += inc8;
@@ -345,6 +368,7 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
         if (base + 4 < n) BM3D_DX_STEP(4)
         if (base + 5 < n) BM3D_DX_STEP(5)
         if (base + 6 < n) BM3D_DX_STEP(6)
+#undef BM3D_DX_GSTEP
 #undef BM3D_DX_STEP
 #undef BM3D_DX_KEY
     }
@@ -352,15 +376,15 @@ __device__ __forceinline__ void dx_rows(const uint32_t* __restrict__ lds, const 
 
 // dx_rows, without the in-plane checks when every lane's window lies inside
 // the plane.
-template <int K, bool kCount, bool kDiv8>
+template <int K, bool kCount, bool kDiv8, bool kIns>
 __device__ __forceinline__ void dx_window(const uint32_t* __restrict__ lds, const PairGeom& g,
                                           const f16x2 (&rp)[8][4], float nr, KeyList<K>& kl, int cx, int cy,
                                           int range, int W, int H, uint32_t idx_base, unsigned& cnt) {
     const bool inside = cx - range >= 0 && cx + range <= W - 8 && cy - range >= 0 && cy + range <= H - 8;
     if (__all(inside))
-        dx_rows<K, kCount, false, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
+        dx_rows<K, kCount, false, kDiv8, kIns>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
     else
-        dx_rows<K, kCount, true, kDiv8>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
+        dx_rows<K, kCount, true, kDiv8, kIns>(lds, g, rp, nr, kl, cx, cy, range, W, H, idx_base, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -455,7 +479,7 @@ __device__ __forceinline__ uint32_t strip_step(const f16x2 (&rs)[SW][4], float n
 
 // Strip scan of the window of radius `range` around the block at (xs, y)
 // (xs: the lane's strip, = its block's x when it owns one).
-template <int K, int SW, bool kWave, bool kCount, bool kCheck>
+template <int K, int SW, bool kWave, bool kCount, bool kCheck, bool kIns>
 __device__ __forceinline__ void strip_rows(const uint32_t* __restrict__ lds, const PairGeom& g,
                                            const f16x2 (&rs)[SW][4], float nrs, KeyList<K>& kl, int xs, int y,
                                            int range, int W, int H, unsigned& cnt) {
@@ -497,18 +521,23 @@ __device__ __forceinline__ void strip_rows(const uint32_t* __restrict__ lds, con
 #define BM3D_ST_KEY(U)                                                                                         \
     strip_step<U, SW, kWave, kCount, kCheck>(rs, nrs, ring, csq, N, ad[U], base + U, vlo, vspan,                 \
                                              rowidx + (uint32_t)(base + U), hi_mask, cnt)
-#define BM3D_ST_STEP(U) kl.push(BM3D_ST_KEY(U));
+#define BM3D_ST_STEP(U)                            \
+    if constexpr (kIns) kl.insert(BM3D_ST_KEY(U));  \
+    else kl.push(BM3D_ST_KEY(U));
+#define BM3D_ST_GSTEP(U)                                     \
+    if constexpr (K > 8 || !kIns) keys[U] = BM3D_ST_KEY(U);  \
+    else kl.insert(BM3D_ST_KEY(U));
         for (; base + 8 <= n; base += 8) {
             uint32_t keys[8];
-            keys[0] = BM3D_ST_KEY(0);
-            keys[1] = BM3D_ST_KEY(1);
-            keys[2] = BM3D_ST_KEY(2);
-            keys[3] = BM3D_ST_KEY(3);
-            keys[4] = BM3D_ST_KEY(4);
-            keys[5] = BM3D_ST_KEY(5);
-            keys[6] = BM3D_ST_KEY(6);
-            keys[7] = BM3D_ST_KEY(7);
-            kl.push8(keys);
+            BM3D_ST_GSTEP(0)
+            BM3D_ST_GSTEP(1)
+            BM3D_ST_GSTEP(2)
+            BM3D_ST_GSTEP(3)
+            BM3D_ST_GSTEP(4)
+            BM3D_ST_GSTEP(5)
+            BM3D_ST_GSTEP(6)
+            BM3D_ST_GSTEP(7)
+            if constexpr (K > 8 || !kIns) kl.push8(keys);
 #pragma unroll
             for (int k = 0; k < 8; ++k) ad[k] += inc8;
         }
@@ -519,6 +548,7 @@ __device__ __forceinline__ void strip_rows(const uint32_t* __restrict__ lds, con
         if (base + 4 < n) BM3D_ST_STEP(4)
         if (base + 5 < n) BM3D_ST_STEP(5)
         if (base + 6 < n) BM3D_ST_STEP(6)
+#undef BM3D_ST_GSTEP
 #undef BM3D_ST_STEP
 #undef BM3D_ST_KEY
     }
@@ -527,7 +557,7 @@ __device__ __forceinline__ void strip_rows(const uint32_t* __restrict__ lds, con
 // The spatial search of the lane's block by strips: reference strip, then
 // the scan, without the in-plane checks when every owned block's window
 // lies inside the plane.
-template <int K, int SW, bool kWave, bool kCount>
+template <int K, int SW, bool kWave, bool kCount, bool kIns>
 __device__ __forceinline__ void strip_window(const uint32_t* __restrict__ lds, const PairGeom& g,
                                              const float* __restrict__ plane, int stride, float mu, KeyList<K>& kl,
                                              int xs, int y, bool own, int range, int W, int H, unsigned& cnt) {
@@ -535,9 +565,9 @@ __device__ __forceinline__ void strip_window(const uint32_t* __restrict__ lds, c
     const float nrs = ref_strip<SW>(rs, plane, stride, W, xs, y, mu);
     const bool inside = !own || (xs - range >= 0 && xs + range <= W - 8 && y - range >= 0 && y + range <= H - 8);
     if (__all(inside))
-        strip_rows<K, SW, kWave, kCount, false>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
+        strip_rows<K, SW, kWave, kCount, false, kIns>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
     else
-        strip_rows<K, SW, kWave, kCount, true>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
+        strip_rows<K, SW, kWave, kCount, true, kIns>(lds, g, rs, nrs, kl, xs, y, range, W, H, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -839,6 +869,10 @@ __device__ __forceinline__ float ref_pairs(f16x2 (&rp)[8][4], const float* __res
 template <bool kCount, int kKt, int kSp, int kPh>
 __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
     constexpr bool kDiv8 = kSp != kSpOther;
+    // key selection by sorted insertion where it pays (dx_rows): in the
+    // two-launch kernels; the one-launch kernel keeps batch merges, which
+    // its register allocation handles better (measured)
+    constexpr bool kIns = kPh != kPhBoth;
     extern __shared__ __align__(16) float lds_f[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds_f);
     __shared__ uint32_t centre_s[256][8];  // the spatial list, per lane (before it: mx_window's exchange)
@@ -924,13 +958,13 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
             if (strip) {
                 const int xs = (bx0 + lx) * a.bs;
                 if constexpr (kSp == kSpStrip4)
-                    strip_window<kKSpatial, 4, false, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                    strip_window<kKSpatial, 4, false, kCount, kIns>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
                                                               a.bm_range, a.W, a.H, cnt);
                 else if constexpr (kSp == kSpStrip4W)
-                    strip_window<kKSpatial, 4, true, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                    strip_window<kKSpatial, 4, true, kCount, kIns>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
                                                              a.bm_range, a.W, a.H, cnt);
                 else if constexpr (kSp == kSpStrip2)
-                    strip_window<kKSpatial, 2, false, kCount>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
+                    strip_window<kKSpatial, 2, false, kCount, kIns>(lds, pg, plane_r, a.stride, mu, kl, xs, y, own,
                                                               a.bm_range, a.W, a.H, cnt);
             } else if (mx) {
                 if constexpr (kSp == kSpMx) {
@@ -949,7 +983,7 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
             } else {
                 f16x2 rp[8][4];
                 const float nr = ref_pairs(rp, plane_r, a.stride, x, y, mu);
-                dx_window<kKSpatial, kCount, kDiv8>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
+                dx_window<kKSpatial, kCount, kDiv8, kIns>(lds, pg, rp, nr, kl, x, y, a.bm_range, a.W, a.H, 0u, cnt);
             }
             Best8Id bi;
             bi.init(kFltMax, pack_pos(x, y, r));
@@ -1016,7 +1050,7 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
                         uint32_t p = last[0];
 #pragma unroll
                         for (int k = 1; k < 8; ++k) p = (i == k) ? last[k] : p;
-                        dx_window<kKt, kCount, kDiv8>(lds, pg, rp, nr, kl, pos_x(p), pos_y(p), a.ps_range, a.W, a.H,
+                        dx_window<kKt, kCount, kDiv8, kIns>(lds, pg, rp, nr, kl, pos_x(p), pos_y(p), a.ps_range, a.W, a.H,
                                                (uint32_t)(i * nWt * nWt), cnt);
                     }
                     Best8Id fi;
