@@ -1121,6 +1121,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
     match_dx_body<kCount, kKt, kSp, kPh>(a, launch_wg());
 }
 
+// The spatial launch of the two-launch strip search: its tile and the
+// spatial search alone fit 4 waves per SIMD (128 VGPRs).
+template <bool kCount, int kSp>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_match_dx_spatial(MatchArgs a) {
+    match_dx_body<kCount, 8, kSp, kPhSpatial>(a, launch_wg());
+}
+
 template <int kKt, int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
@@ -1135,8 +1142,13 @@ hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, 
     BM3D_INJECT_LAUNCH();
     dim3 grid((a.nbx + a.obw - 1) / a.obw, (a.nby + tby - 1) / tby, frames);
     dim3 block(a.tbw * tby);
-    if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
-    else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
+    if constexpr (kPh == kPhSpatial) {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx_spatial<true, kSp>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_match_dx_spatial<false, kSp>), grid, block, lds_bytes, s, a);
+    } else {
+        if (a.ncand) hipLaunchKernelGGL((bm3d_match_dx<true, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
+        else hipLaunchKernelGGL((bm3d_match_dx<false, kKt, kSp, kPh>), grid, block, lds_bytes, s, a);
+    }
     return hipGetLastError();
 }
 template <int kKt, int kSp>
@@ -1153,12 +1165,18 @@ hipError_t set_dx_limit(size_t bytes) {
                          reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSp, kPhBoth>),
                          reinterpret_cast<const void*>(&bm3d_match_dx_multi<kKt, kSp>),
                          reinterpret_cast<const void*>(&bm3d_match_dx<false, kKt, kSpLane, kPhTemporal>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSpLane, kPhTemporal>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<false, 8, kSp, kPhSpatial>),
-                         reinterpret_cast<const void*>(&bm3d_match_dx<true, 8, kSp, kPhSpatial>)};
+                         reinterpret_cast<const void*>(&bm3d_match_dx<true, kKt, kSpLane, kPhTemporal>)};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
+    }
+    if constexpr (kSp >= kSpStrip4) {  // the spatial launches exist for the strip searches
+        const void* sp[] = {reinterpret_cast<const void*>(&bm3d_match_dx_spatial<false, kSp>),
+                            reinterpret_cast<const void*>(&bm3d_match_dx_spatial<true, kSp>)};
+        for (const void* f : sp) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }
