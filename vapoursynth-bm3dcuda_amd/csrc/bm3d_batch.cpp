@@ -51,7 +51,17 @@ struct Job {
     int wave = 0;
     // scratch of the job within its wave's arena (bytes)
     size_t corner_off = 0, wrap_off = 0, range_off = 0, groups_off = 0, scratch_off = 0;
+    size_t slist_off = 0;  // spatial lists between the two matching launches (two_launch_match)
 };
+
+// Whether a job's matching runs as the two-launch strip search (spatial by
+// strips, then temporal with one lane per block, bm3d_capi.cpp
+// launch_matching), for either pass.
+bool two_launch_match(const Job& j) {
+    for (int pass = 0; pass < j.job.passes; ++pass)
+        if (bm3d::plan_uses_strip(j.plan[pass]) && j.plan[pass]->d.radius > 0) return true;
+    return false;
+}
 
 // One kernel launch of the graph.
 struct Launch {
@@ -60,6 +70,7 @@ struct Launch {
     int pass = 0;
     int block = 0;  // threads (matching) or waves (filtering)
     int keys = 0, final_ = 0, chroma = 0;  // keys: the mixed matcher's temporal keys
+    int phase = 0;                         // mixed matching: MatchArgs::phase
     int any_extractor = 0;                 // mixed filtering: some job has extractor_exp
     int variant = 0;  // matching: block distance (FP32) or dx_spatial_variant (mixed);
                       // filtering: the FP32 generic-transform variant
@@ -136,6 +147,10 @@ void layout(bm3d_batch* b) {
             off += align_up((size_t)j.job.src.num_frames * j.plan[0]->nblocks * 8 * sizeof(uint32_t), 256);
             j.scratch_off = off;
             off += align_up((size_t)j.job.src.num_frames * j.plan[0]->region_ints * sizeof(int), 256);
+            if (two_launch_match(j)) {
+                j.slist_off = off;
+                off += align_up((size_t)j.job.src.num_frames * j.plan[0]->nblocks * 16 * sizeof(uint32_t), 256);
+            }
         }
         b->wave_zero.emplace_back(0, zero_bytes);
         b->arena_bytes = std::max(b->arena_bytes, off);
@@ -148,7 +163,8 @@ void layout(bm3d_batch* b) {
         const bm3d_plan* p = j.plan[0];
         const long long N = j.job.src.num_frames;
         const size_t need = (size_t)N * ((size_t)j.nplanes * p->plane_size * 8 + (size_t)p->nblocks * 32 +
-                                         (size_t)p->region_ints * 4) + 4096;
+                                         (size_t)p->region_ints * 4 + (two_launch_match(j) ? p->nblocks * 64 : 0)) +
+                            4096;
         if (!wave_jobs.empty() && used + need > (size_t)b->wave_budget) close_wave();
         j.wave = b->nwaves;
         wave_jobs.push_back(i);
@@ -215,19 +231,27 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
             // block matching, grouped by kernel variant and block size (the
             // variant: the distance for the FP32 matcher, the spatial search
             // for the mixed one)
-            std::map<std::tuple<int, int, int, int>, std::vector<int>> mgroups;
+            // (the two-launch strip searches as a spatial launch, phase 1, then
+            // a temporal one, phase 2, each over all such jobs)
+            std::map<std::tuple<int, int, int, int, int>, std::vector<int>> mgroups;
             for (int i : js) {
                 const bm3d_plan* p = b->jobs[i].plan[pass];
                 const int keys = p->use_dx ? bm3d::plan_temporal_keys(p) : 0;
                 const int variant = p->use_dx ? bm3d::plan_dx_variant(p) : p->d.bm_error;
-                mgroups[{p->use_dx ? 1 : 0, keys, p->tbw * p->tby, variant}].push_back(i);
+                if (p->use_dx && bm3d::plan_uses_strip(p) && p->d.radius > 0) {
+                    mgroups[{1, bm3d::kPhSpatial, 0, p->tbw * p->tby, variant}].push_back(i);
+                    mgroups[{1, bm3d::kPhTemporal, keys, p->tbw * p->tby, bm3d::kSpLane}].push_back(i);
+                } else {
+                    mgroups[{p->use_dx ? 1 : 0, bm3d::kPhBoth, keys, p->tbw * p->tby, variant}].push_back(i);
+                }
             }
             for (auto& [key, ids] : mgroups) {
                 Launch l{};
                 l.kind = std::get<0>(key) ? 2 : 1;
                 l.pass = pass;
-                l.block = std::get<2>(key);
-                l.variant = std::get<3>(key);
+                l.phase = std::get<1>(key);
+                l.block = std::get<3>(key);
+                l.variant = std::get<4>(key);
                 std::vector<bm3d::MatchArgs> args;
                 std::vector<bm3d::JobGrid> grids;
                 for (int i : ids) {
@@ -237,7 +261,13 @@ int build_launches(bm3d_batch* b, std::vector<char>& tab) {
                     bm3d::MatchArgs a = bm3d::plan_match_args(p, &j.job.src, &ref);
                     a.groups = reinterpret_cast<uint32_t*>(base + j.groups_off);
                     a.first_frame = 0;
-                    const int obw = bm3d::plan_match_owned_width(p);
+                    a.phase = l.phase;
+                    if (l.phase != bm3d::kPhBoth) a.slist = reinterpret_cast<uint32_t*>(base + j.slist_off);
+                    if (l.phase == bm3d::kPhTemporal) {  // one lane per block
+                        a.strip = 0;
+                        a.obw = a.tbw;
+                    }
+                    const int obw = a.obw;
                     const int gx = (p->nbx + obw - 1) / obw, gy = (p->nby + p->tby - 1) / p->tby;
                     grids.push_back({l.wgs, gx, gy});
                     l.wgs += gx * gy * j.job.src.num_frames;
@@ -355,7 +385,7 @@ hipError_t enqueue(const bm3d_batch* b, const Launch& l, hipStream_t s) {
                                             l.wgs, l.block, l.lds, l.variant, s);
         case 2:
             return bm3d::launch_match_dx_multi(reinterpret_cast<const bm3d::MatchArgs*>(t + l.args_off), grids, n,
-                                               l.wgs, l.block, l.lds, l.keys, l.variant, s);
+                                               l.wgs, l.block, l.lds, l.keys, l.variant, s, l.phase);
         case 3:
             return bm3d::launch_filter_multi(reinterpret_cast<const bm3d::FilterArgs*>(t + l.args_off), grids, n,
                                              l.wgs, l.final_ != 0, l.chroma != 0, l.variant != 0, l.block, l.lds, s);

@@ -215,10 +215,10 @@ hipError_t launch_match(const MatchArgs& a, int tby, size_t lds_bytes, int frame
 // variant; lds_bytes covers the largest job.
 hipError_t launch_match_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
                               size_t lds_bytes, int cost, hipStream_t s);
-// (every job of one launch has the same MatchArgs::kt, and `variant` is
-// dx_spatial_variant of each job)
+// (every job of one launch has the same MatchArgs::kt and phase, and
+// `variant` is dx_spatial_variant of each job)
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int kt, int variant, hipStream_t s);
+                                 size_t lds_bytes, int kt, int variant, hipStream_t s, int phase = 0);
 // generic: the FP32 filter variant for transform kinds other than the DCT.
 hipError_t launch_filter_multi(const FilterArgs* jobs, const JobGrid* grids, int njobs, int wgs, bool final_,
                                bool chroma, bool generic, int waves, size_t lds_bytes, hipStream_t s);

@@ -1136,6 +1136,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
     match_dx_body<false, kKt, kSp, kPhBoth>(a, job_wg(grids[j], blockIdx.x));
 }
 
+// Multi-job forms of the two-launch strip search's launches.
+template <int kSp>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_match_dx_spatial_multi(
+    const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const MatchArgs& a = jobs[j];
+    match_dx_body<false, 8, kSp, kPhSpatial>(a, job_wg(grids[j], blockIdx.x));
+}
+
+template <int kKt>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_temporal_multi(
+    const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
+    const int j = find_job(grids, njobs, blockIdx.x);
+    const MatchArgs& a = jobs[j];
+    match_dx_body<false, kKt, kSpLane, kPhTemporal>(a, job_wg(grids[j], blockIdx.x));
+}
+
 namespace {
 template <int kKt, int kSp, int kPh = kPhBoth>
 hipError_t launch_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s) {
@@ -1172,7 +1189,9 @@ hipError_t set_dx_limit(size_t bytes) {
     }
     if constexpr (kSp >= kSpStrip4) {  // the spatial launches exist for the strip searches
         const void* sp[] = {reinterpret_cast<const void*>(&bm3d_match_dx_spatial<false, kSp>),
-                            reinterpret_cast<const void*>(&bm3d_match_dx_spatial<true, kSp>)};
+                            reinterpret_cast<const void*>(&bm3d_match_dx_spatial<true, kSp>),
+                            reinterpret_cast<const void*>(&bm3d_match_dx_spatial_multi<kSp>),
+                            reinterpret_cast<const void*>(&bm3d_match_dx_temporal_multi<kKt>)};
         for (const void* f : sp) {
             hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
             if (e != hipSuccess) return e;
@@ -1243,8 +1262,31 @@ hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int fr
 }
 
 hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, int njobs, int wgs, int threads,
-                                 size_t lds_bytes, int kt, int variant, hipStream_t s) {
+                                 size_t lds_bytes, int kt, int variant, hipStream_t s, int phase) {
     BM3D_INJECT_LAUNCH();
+    const dim3 g(wgs), blk(threads);
+    if (phase == kPhSpatial) {
+        switch (variant) {
+            case kSpStrip4W:
+                hipLaunchKernelGGL((bm3d_match_dx_spatial_multi<kSpStrip4W>), g, blk, lds_bytes, s, jobs, grids, njobs);
+                break;
+            case kSpStrip2:
+                hipLaunchKernelGGL((bm3d_match_dx_spatial_multi<kSpStrip2>), g, blk, lds_bytes, s, jobs, grids, njobs);
+                break;
+            default:
+                hipLaunchKernelGGL((bm3d_match_dx_spatial_multi<kSpStrip4>), g, blk, lds_bytes, s, jobs, grids, njobs);
+        }
+        return hipGetLastError();
+    }
+    if (phase == kPhTemporal) {
+        switch (kt) {
+            case 4: hipLaunchKernelGGL((bm3d_match_dx_temporal_multi<4>), g, blk, lds_bytes, s, jobs, grids, njobs); break;
+            case 8: hipLaunchKernelGGL((bm3d_match_dx_temporal_multi<8>), g, blk, lds_bytes, s, jobs, grids, njobs); break;
+            default:
+                hipLaunchKernelGGL((bm3d_match_dx_temporal_multi<16>), g, blk, lds_bytes, s, jobs, grids, njobs);
+        }
+        return hipGetLastError();
+    }
     switch (variant) {
         case kSpOther: return launch_dx_multi_keys<kSpOther>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
         case kSpMx: return launch_dx_multi_keys<kSpMx>(jobs, grids, njobs, wgs, threads, lds_bytes, kt, s);
