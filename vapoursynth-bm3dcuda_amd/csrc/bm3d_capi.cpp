@@ -729,9 +729,9 @@ int plan_temporal_keys(const bm3d_plan* p) {
 bool plan_uses_strip(const bm3d_plan* p) {
     return p->use_dx && p->strip && !p->mx_spatial && p->lds_bytes && strip_supported(p->tbw, p->d.block_step);
 }
-// Batches run the whole search in one launch over all their jobs (at radius
-// > 0 the strip search's last lanes then idle through the temporal search;
-// the lists are the two-launch form's).
+// A batch's matching arguments describe the strip search wherever the plan
+// uses it (at radius > 0 the batch turns them into its two launches, as
+// launch_matching does).
 bool plan_batch_strip(const bm3d_plan* p) { return plan_uses_strip(p); }
 int plan_match_owned_width(const bm3d_plan* p) {
     return plan_batch_strip(p) ? strip_owned_width(p->tbw, p->d.block_step) : p->tbw;

@@ -100,10 +100,11 @@ MatchArgs plan_match_args(const bm3d_plan* p, const bm3d_clip* src, const bm3d_c
 // or dx_temporal_keys(ps_num) when that is 0.
 int plan_temporal_keys(const bm3d_plan* p);
 // Whether the plan's matching uses the strip search (at radius > 0 as two
-// launches, spatial then temporal); whether a batch's one-launch matching
-// does (as the plan's, with the same lists); the blocks a batch's matching work-group row owns
-// (tbw without strips); the batch's dx_spatial_variant of the plan.
-// plan_match_args describes the batch's launches.
+// launches, spatial then temporal); whether a batch's matching arguments
+// describe it (as the plan's, with the same lists); the blocks a batch's
+// matching work-group row owns (tbw without strips); the batch's
+// dx_spatial_variant of the plan. plan_match_args describes the batch's
+// launches (the batch clears the strip fields for a temporal launch).
 bool plan_uses_strip(const bm3d_plan* p);
 bool plan_batch_strip(const bm3d_plan* p);
 int plan_match_owned_width(const bm3d_plan* p);
