@@ -472,8 +472,16 @@ hipError_t launch_matching(bm3d_plan* p, uint32_t* groups, unsigned long long* n
                 mt.strip = 0;
                 mt.obw = mt.tbw;
                 mt.groups = groups ? groups + (size_t)c0 * p->nblocks * 8 : nullptr;
-                e = bm3d::launch_match_dx(ms, p->tby, p->dx_lds_bytes, nf, stream);
-                if (e == hipSuccess) e = bm3d::launch_match_dx(mt, p->tby, p->dx_lds_bytes, nf, stream);
+                // through the launch table (fewer scalar registers); the
+                // counting launches take the plain kernels
+                if (ncand) {
+                    e = bm3d::launch_match_dx(ms, p->tby, p->dx_lds_bytes, nf, stream);
+                    if (e == hipSuccess) e = bm3d::launch_match_dx(mt, p->tby, p->dx_lds_bytes, nf, stream);
+                } else {
+                    e = bm3d::launch_match_dx_table(ms, p->tby, p->dx_lds_bytes, nf, p->match_table, stream);
+                    if (e == hipSuccess)
+                        e = bm3d::launch_match_dx_table(mt, p->tby, p->dx_lds_bytes, nf, p->match_table, stream);
+                }
             }
             return e;
         }
@@ -859,13 +867,15 @@ int bm3d_plan_create(const bm3d_desc* desc, bm3d_plan** out) {
     // the strip search: about a fifth fewer instructions in the spatial search
     p->strip = true;
     p->vbound = bm3d::contribution_bound(d.transform_2d, d.transform_1d, p->d.final_ != 0);
-    e = bm3d::malloc_device(&p->range, 256);  // the range, then the wrap flag
+    // the range, then the wrap flag; from byte 256 the matching launch table
+    e = bm3d::malloc_device(&p->range, 256 + bm3d::kMatchTableBytes);
     if (e == hipSuccess) e = hipMemset(p->range, 0, 256);
     if (e != hipSuccess) {
         bm3d_plan_destroy(p);
         return hip_fail(e, "hipMalloc");
     }
     p->wrap_flag = reinterpret_cast<int*>(p->range) + 1;
+    p->match_table = reinterpret_cast<char*>(p->range) + 256;
 
     if (d.precision == BM3D_PRECISION_MIXED) {
         std::vector<uint32_t> ops((size_t)bm3d::kMxOpVectors * 64 * 4);

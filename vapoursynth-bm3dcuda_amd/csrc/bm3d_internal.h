@@ -75,6 +75,8 @@ struct bm3d_plan {
     // slist (16 words per block and frame, slist_cap words)
     uint32_t* slist = nullptr;
     size_t slist_cap = 0;
+    // kMatchTableBytes of device memory after the range (launch_match_dx_table)
+    void* match_table = nullptr;
     int tile_hp2 = 0;
     size_t dx_lds_bytes = 0;
     // mixed-precision filter kernel (precision MIXED): per-lane transform operands

@@ -241,6 +241,15 @@ hipError_t set_match_lds_limit(size_t bytes);
 // launch_match; LDS bytes as returned by dx_lds_bytes.
 hipError_t launch_match_dx(const MatchArgs& a, int tby, size_t lds_bytes, int frames, hipStream_t s);
 hipError_t set_match_dx_lds_limit(size_t bytes);
+// launch_match_dx through the multi-job kernels, one job whose arguments a
+// small kernel first stores in `table` (device memory of kMatchTableBytes):
+// reading the arguments from memory instead of the kernel-argument segment,
+// the compiler keeps fewer of them in scalar registers (the temporal search
+// spills 76 instead of 172 B per lane). Not for counting launches.
+constexpr size_t kMatchTableGridOffset = (sizeof(MatchArgs) + 63) / 64 * 64;
+constexpr size_t kMatchTableBytes = kMatchTableGridOffset + 64;
+hipError_t launch_match_dx_table(const MatchArgs& a, int tby, size_t lds_bytes, int frames, void* table,
+                                 hipStream_t s);
 // Default keys kept per temporal frame: 8 (16 for ps_num above 4).
 int dx_temporal_keys(int ps_num);
 // The mixed matcher's kernel variants, by spatial search: per-lane scans

@@ -1297,6 +1297,30 @@ hipError_t launch_match_dx_multi(const MatchArgs* jobs, const JobGrid* grids, in
     }
 }
 
+namespace {
+// Writes one launch's arguments and its one-job grid where a multi-job kernel
+// reads them, in stream order (a kernel node, so graph captures hold it).
+__global__ void bm3d_store_match_table(MatchArgs a, JobGrid g, MatchArgs* __restrict__ da, JobGrid* __restrict__ dg) {
+    if (threadIdx.x == 0) {
+        *da = a;
+        *dg = g;
+    }
+}
+}  // namespace
+
+hipError_t launch_match_dx_table(const MatchArgs& a, int tby, size_t lds_bytes, int frames, void* table,
+                                 hipStream_t s) {
+    BM3D_INJECT_LAUNCH();
+    MatchArgs* da = static_cast<MatchArgs*>(table);
+    JobGrid* dg = reinterpret_cast<JobGrid*>(static_cast<char*>(table) + kMatchTableGridOffset);
+    const int gx = (a.nbx + a.obw - 1) / a.obw, gy = (a.nby + tby - 1) / tby;
+    hipLaunchKernelGGL(bm3d_store_match_table, dim3(1), dim3(64), 0, s, a, JobGrid{0, gx, gy}, da, dg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_match_dx_multi(da, dg, 1, gx * gy * frames, a.tbw * tby, lds_bytes, a.kt, dx_spatial_variant(a), s,
+                                 a.phase);
+}
+
 hipError_t set_match_dx_lds_limit(size_t bytes) {
     hipError_t e = hipSuccess;
     for (hipError_t (*f)(size_t) : {set_dx_limits<kSpLane>, set_dx_limits<kSpOther>, set_dx_limits<kSpMx>,
