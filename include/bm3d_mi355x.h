@@ -201,8 +201,11 @@ long long bm3d_num_blocks(const bm3d_plan *plan);
 /* Block-matching kernel of the plan: 1 for the mixed-precision kernel
  * (precision MIXED: FP16 pre-scoring, exact FP32 rescoring of the best few)
  * with a per-lane spatial search, 3 for the same kernel with its spatial
- * search by strips shared between neighbouring lanes (the default for block
- * steps 2 and 4 at radius 0), 2 for the same kernel with the spatial search's cross
+ * search by strips shared between neighbouring lanes (the default at block
+ * step 4, and at block step 2 with 16-lane rows; at radius > 0 it runs as two
+ * launches, the spatial search by strips and then the temporal search with
+ * one lane per block, handing 64 B per block over in a buffer of the plan;
+ * bm3d_execute_noalloc runs both in one launch), 2 for the same kernel with the spatial search's cross
  * terms on the matrix cores where the work-group tile allows (opt-in: slower
  * on MI355X), 0 for the FP32 kernel (precision FP32, or a search window too
  * large for the mixed kernel's 12-bit scan indices). No counterpart in the
