@@ -168,6 +168,39 @@ struct Wg {
 };
 __device__ __forceinline__ Wg launch_wg() { return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z}; }
 
+// Work-groups are dispatched to the 8 XCDs of an MI355X in turn (flat id L
+// runs on XCD L % 8), and each XCD has its own L2. xcd_order renumbers them
+// so that each XCD takes runs of kXcdRun consecutive ids, the runs dealt to
+// the XCDs in turn: tiles that neighbour in the grid, which read
+// overlapping rows and frames, then meet in one L2, while every XCD still
+// gets an even share of each part of the grid (multi-job launches mix
+// light and heavy jobs). The ids past the last whole round of runs are
+// split into 8 contiguous shares. A permutation of [0, T), so every
+// work-group still runs once. The block matchers use it (their search
+// windows overlap from tile to tile); the filters and VAggregate keep the
+// plain order, which measured faster for them.
+constexpr int kXcdRun = 128;
+__device__ __forceinline__ int xcd_order(int L, int T) {
+    constexpr int kRound = 8 * kXcdRun;
+    const int whole = T - T % kRound;
+    if (L < whole) {
+        const int o = L % kRound;
+        return L - o + (o & 7) * kXcdRun + (o >> 3);
+    }
+    const int t = T - whole, l = L - whole;
+    const int q = t >> 3, r = t & 7, k = l & 7;
+    return whole + k * q + min(k, r) + (l >> 3);
+}
+// launch_wg in XCD order
+__device__ __forceinline__ Wg launch_wg_xcd() {
+    const int gx = gridDim.x, gxy = gx * gridDim.y;
+    const int n = xcd_order(blockIdx.x + gx * blockIdx.y + gxy * blockIdx.z, gxy * gridDim.z);
+    const int z = n / gxy, rem = n - z * gxy, y = rem / gx;
+    return {rem - y * gx, y, z};
+}
+// flat work-group index of a multi-job launch in XCD order
+__device__ __forceinline__ int multi_wg_xcd() { return xcd_order(blockIdx.x, gridDim.x); }
+
 // Job of flat work-group index wg (binary search over the sorted begins;
 // every value is uniform over the work-group, so this runs on the scalar
 // unit).

@@ -284,15 +284,16 @@ __device__ __forceinline__ void match_body(const MatchArgs& a, Wg w) {
 
 template <int kCost, bool kCount, bool kPlane>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs a) {
-    match_body<kCost, kCount, kPlane>(a, launch_wg());
+    match_body<kCost, kCount, kPlane>(a, launch_wg_xcd());
 }
 
 template <int kCost>
 __global__ __launch_bounds__(256) void bm3d_match_multi(const MatchArgs* __restrict__ jobs,
                                                         const JobGrid* __restrict__ grids, int njobs) {
-    const int j = find_job(grids, njobs, blockIdx.x);
+    const int wg = multi_wg_xcd();
+    const int j = find_job(grids, njobs, wg);
     const MatchArgs& a = jobs[j];
-    match_body<kCost, false, false>(a, job_wg(grids[j], blockIdx.x));
+    match_body<kCost, false, false>(a, job_wg(grids[j], wg));
 }
 
 // ---------------------------------------------------------------- filtering

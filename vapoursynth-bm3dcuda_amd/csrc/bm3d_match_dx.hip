@@ -1118,39 +1118,42 @@ __device__ __forceinline__ void match_dx_body(const MatchArgs& a, Wg w) {
 // the part of the search (match_dx_body).
 template <bool kCount, int kKt, int kSp, int kPh>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx(MatchArgs a) {
-    match_dx_body<kCount, kKt, kSp, kPh>(a, launch_wg());
+    match_dx_body<kCount, kKt, kSp, kPh>(a, launch_wg_xcd());
 }
 
 // The spatial launch of the two-launch strip search: its tile and the
 // spatial search alone fit 4 waves per SIMD (128 VGPRs).
 template <bool kCount, int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_match_dx_spatial(MatchArgs a) {
-    match_dx_body<kCount, 8, kSp, kPhSpatial>(a, launch_wg());
+    match_dx_body<kCount, 8, kSp, kPhSpatial>(a, launch_wg_xcd());
 }
 
 template <int kKt, int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
-    const int j = find_job(grids, njobs, blockIdx.x);
+    const int wg = multi_wg_xcd();
+    const int j = find_job(grids, njobs, wg);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, kKt, kSp, kPhBoth>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, kKt, kSp, kPhBoth>(a, job_wg(grids[j], wg));
 }
 
 // Multi-job forms of the two-launch strip search's launches.
 template <int kSp>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bm3d_match_dx_spatial_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
-    const int j = find_job(grids, njobs, blockIdx.x);
+    const int wg = multi_wg_xcd();
+    const int j = find_job(grids, njobs, wg);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, 8, kSp, kPhSpatial>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, 8, kSp, kPhSpatial>(a, job_wg(grids[j], wg));
 }
 
 template <int kKt>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void bm3d_match_dx_temporal_multi(
     const MatchArgs* __restrict__ jobs, const JobGrid* __restrict__ grids, int njobs) {
-    const int j = find_job(grids, njobs, blockIdx.x);
+    const int wg = multi_wg_xcd();
+    const int j = find_job(grids, njobs, wg);
     const MatchArgs& a = jobs[j];
-    match_dx_body<false, kKt, kSpLane, kPhTemporal>(a, job_wg(grids[j], blockIdx.x));
+    match_dx_body<false, kKt, kSpLane, kPhTemporal>(a, job_wg(grids[j], wg));
 }
 
 namespace {
