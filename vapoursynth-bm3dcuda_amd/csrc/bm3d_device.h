@@ -179,7 +179,7 @@ __device__ __forceinline__ Wg launch_wg() { return {(int)blockIdx.x, (int)blockI
 // work-group still runs once. The block matchers use it (their search
 // windows overlap from tile to tile); the filters and VAggregate keep the
 // plain order, which measured faster for them.
-constexpr int kXcdRun = 128;
+constexpr int kXcdRun = 64;
 __device__ __forceinline__ int xcd_order(int L, int T) {
     constexpr int kRound = 8 * kXcdRun;
     const int whole = T - T % kRound;
