@@ -237,13 +237,17 @@ __device__ __forceinline__ void filter_mx_body(const FilterArgs& a, Wg wg) {
         // member rows it addresses are requested as soon as the step starts
         // (the compiler cannot move the load above the previous step's
         // global atomics itself). Prefetching the rows too measured slower.
+        // (li / tbx through a float reciprocal: exact for li < 2^20)
+        const float inv_tbx = 1.0f / (float)a.tbx;
+        const uint32_t* gbase = a.groups + (size_t)gf * a.nblocks * 8 + (c & 7);
         auto entry = [&](int base, bool& act) -> uint32_t {
             const int li = base + wave * 2 + (c >> 3);
-            const int bx = tx * a.tbx + li % a.tbx;
-            const int by = ty * a.tby + li / a.tbx;
+            const int qy = (int)(((float)li + 0.5f) * inv_tbx);
+            const int bx = tx * a.tbx + (li - qy * a.tbx);
+            const int by = ty * a.tby + qy;
             act = li < ngroups && bx < a.nbx && by < a.nby;
             const int blk = act ? by * a.nbx + bx : 0;
-            return a.groups[((size_t)gf * a.nblocks + blk) * 8 + (c & 7)];
+            return gbase[(size_t)blk * 8];
         };
         bool next_active = false;
         uint32_t next_pk = entry(0, next_active);

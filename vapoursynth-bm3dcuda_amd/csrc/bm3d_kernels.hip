@@ -840,7 +840,7 @@ __device__ __forceinline__ void vagg_targets_body(const VTargetArgs& a, Wg w) {
         rowsum[rr][tx] = s;
     }
     __syncthreads();
-#pragma unroll 4
+#pragma unroll
     for (int h = 0; h < kR / 4; ++h) {
         const int jj = ty + 4 * h;
         const int y = y0 + jj;
