@@ -569,9 +569,10 @@ def main():
             "compute_pipe": "VALU: FP16 v_dot2 pre-scores (mixed) / FP32 FMA (fp32); peak = MI355X "
                             "published vector peak (FP32 with packed math, = FP16 dot2 rate); no MFMA",
             # the mixed path's matching is two launches per evaluation batch
-            # (the spatial search by strips, then the temporal search); a
-            # "launch" here is the pair, timed together
-            "kernel": ("bm3d_match_dx_spatial+bm3d_match_dx" if main_r["two_launch_match"] else
+            # (the spatial search by strips, then the temporal search, each
+            # a one-job multi-job kernel reading its arguments from a table);
+            # a "launch" here is the pair, timed together
+            "kernel": ("bm3d_match_dx_spatial_multi+bm3d_match_dx_temporal_multi" if main_r["two_launch_match"] else
                        "bm3d_match_dx" if args.precision == "mixed" else "bm3d_match"),
             "achieved": round(main_r["achieved_tflops"], 3),
             "peak": PEAK_FP32_TFLOPS,
